@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark of the XAMG solve path on one MI355X.
+
+Workload (headline, one GPU): 3D 7-point Poisson 128^3 (2,097,152 rows, 14,581,760 nnz),
+CSR with int32 indices, nv=8 right-hand sides uniform[-1,1] from seed 2, x0 = 0,
+PBiCGStab + Chebyshev(4) to a 1e-10 relative residual (abs_tolerance 0).
+
+A step is one complete solve to tolerance with the FP32-inner / FP64-refined method
+(IterativeRefinement around an FP32 PBiCGStab + FP32 Chebyshev), inputs resident in
+HBM.  The plain FP64 GPU solve of the same system is timed the same way; `value` is
+the mixed-precision time to tolerance in seconds (lower is better) and the line also
+carries the FP64 time, the speedup, the final FP64-verified residuals and the error
+against an FP64 reference solution, the SpMV bandwidth with its roofline record, and
+the CPU baseline (the oracle's OpenMP restatement of the reference path, timed on a
+bounded sample of iterations on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "xamg-test_amd")]
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(n, nnz, nv, s_val, s_vec):
+    """Algorithmic SpMV bytes: row pointers, columns+values once, x read once, y written."""
+    return 4 * (n + 1) + nnz * (4 + s_val) + 2 * n * nv * s_vec
+
+
+def solver_params(kind, mixed, inner_rtol, tol):
+    if kind == "cheb4":
+        prec = {"method": "Chebyshev", "polynomial_order": 4, "spectrum_fraction": 0.3}
+        method = "PBiCGStab"
+    elif kind == "jacobi":
+        prec = {"method": "Jacobi"}
+        method = "PBiCGStab"
+    else:
+        prec = None
+        method = "BiCGStab"
+    if not mixed:
+        p = {"solver": {"method": method, "rel_tolerance": tol, "abs_tolerance": 0.0,
+                        "max_iters": 5000, "convergence_details": "check-L2"}}
+    else:
+        p = {"meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
+                             "max_iters": 50, "rel_tolerance": tol, "abs_tolerance": 0.0,
+                             "convergence_details": "check-L2"},
+             "solver": {"method": method, "rel_tolerance": inner_rtol, "abs_tolerance": 0.0,
+                        "max_iters": 5000, "convergence_details": "check-L2"}}
+    if prec is not None:
+        p["preconditioner"] = prec
+    return p
+
+
+def cpu_baseline(row, col, val, b, nv, kind, gpu_iters, eigs, budget_s=15.0):
+    """Oracle (OpenMP C restatement of the reference FP64 path) on the host cores: a
+    bounded number of iterations is timed and scaled to the GPU-measured iteration
+    count of the same FP64 solve."""
+    import oracle as O
+    threads = O.num_threads()
+    A = O.csr(row, col, val)
+    params = solver_params(kind, False, None, 1e-10)
+    probe_iters = 2
+    params["solver"]["max_iters"] = probe_iters
+    tree = O.Tree(params, nchunks=threads, eigs=eigs)
+    s = O.Solve(tree, A, nv)
+    t0 = time.perf_counter()
+    s.run(b)
+    t_probe = time.perf_counter() - t0
+    per_iter = t_probe / probe_iters
+    iters = int(max(1, min(gpu_iters, budget_s / max(per_iter, 1e-9))))
+    params["solver"]["max_iters"] = iters
+    tree = O.Tree(params, nchunks=threads, eigs=eigs)
+    s = O.Solve(tree, A, nv)
+    t0 = time.perf_counter()
+    r = s.run(b)
+    t = time.perf_counter() - t0
+    per_iter = t / max(r["iters"], 1)
+    return {"value": per_iter * gpu_iters, "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 {kind} solve, {r['iters']} iterations timed "
+                      f"({t:.1f} s), scaled to the {gpu_iters} iterations of the GPU FP64 solve"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--nv", type=int, default=8)
+    ap.add_argument("--solver", default="cheb4", choices=["cheb4", "jacobi", "bicgstab"])
+    ap.add_argument("--inner-rtol", type=float, default=1e-5)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build targets one GPU; --gpus N with N > 1 is not supported "
+              "(the path would shard by z-slab row blocks, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import pyxamg as X
+    X.init()
+    X.set_engine(batch=args.batch)
+    dev = X.device_info()
+
+    n1 = args.n
+    nv = args.nv
+    row, col, val = X.poisson7(n1)
+    n, nnz = n1 ** 3, len(col)
+    b = X.rhs(n, nv, 2)
+    A = X.Matrix(row, col, val)
+    c = np.cos(np.pi / (n1 + 1))
+    eigs = (1.0 - c, 1.0 + c)  # closed form for the 7-point Dirichlet Laplacian
+    A.set_eigs(*eigs)
+    xv, bv = X.Vector(n, nv), X.Vector(n, nv)
+    bv.upload(b)
+
+    def make(mixed):
+        P = X.Params(solver_params(args.solver, mixed, args.inner_rtol, args.tol))
+        P.set_default()
+        return X.Solver(A, P, xv, bv)
+
+    def run(S, k):
+        times = []
+        for _ in range(k):
+            xv.set_val(0.0)
+            X.check(X.lib().XAMG_barrier())
+            t0 = time.perf_counter()
+            S.solve()
+            X.check(X.lib().XAMG_barrier())
+            times.append(time.perf_counter() - t0)
+        return times
+
+    S64 = make(False)
+    run(S64, args.warmup)
+    t64 = run(S64, args.steps)
+    it64 = S64.iters()[0]
+    conv64 = S64.converged()
+    x64 = xv.download()
+
+    Smx = make(True)
+    run(Smx, args.warmup)
+    tmx = run(Smx, args.steps)
+    itmx, submx = Smx.iters()
+    convmx = Smx.converged()
+    xmx = xv.download()
+
+    # FP64 verification of both results with the library's FP64 SpMV
+    def relres(x):
+        ax = A.spmv(x, nv, prec=0)
+        r = (b - ax).reshape(n, nv)
+        return (np.linalg.norm(r, axis=0) / np.linalg.norm(b.reshape(n, nv), axis=0)).max()
+
+    rr64, rrmx = relres(x64), relres(xmx)
+    err_mx = np.linalg.norm(xmx - x64) / np.linalg.norm(x64)
+
+    # SpMV roofline of the dominant kernel (FP32 SpMV in the mixed solve, FP64 beside it)
+    iters_spmv = 200
+    ms32 = A.bench_spmv(nv, prec=1, iters=iters_spmv)
+    ms64 = A.bench_spmv(nv, prec=0, iters=iters_spmv)
+    b32 = spmv_bytes(n, nnz, nv, 4, 4)
+    b64 = spmv_bytes(n, nnz, nv, 8, 8)
+    gbs32 = b32 / (ms32 * 1e-3) / 1e9
+    gbs64 = b64 / (ms64 * 1e-3) / 1e9
+    copy_gbs = X.bench_copy(2 << 30, iters=20)
+
+    mean_mx = float(np.mean(tmx))
+    mean_64 = float(np.mean(t64))
+    line = {
+        "metric": "time-to-1e-10 residual, FP32+refine vs FP64, 7pt 128^3 nv=8; SpMV HBM GB/s",
+        "value": mean_mx,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mean_mx * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-outer",
+        "data": "synthetic (7-point Poisson generator, pseudo_rand RHS seed 2)",
+        "config": {"workload": f"7pt-poisson {n1}^3 nv={nv} {args.solver} rtol={args.tol}",
+                   "rows": n, "nnz": nnz, "nv": nv, "solver": args.solver,
+                   "inner_rel_tolerance": args.inner_rtol, "parallelism": "1 GPU"},
+        "fp64_time_s": mean_64,
+        "speedup_vs_fp64": mean_64 / mean_mx,
+        "fp64_iters": it64,
+        "mixed_outer_iters": itmx,
+        "mixed_inner_iters": submx,
+        "converged": bool(all(conv64) and all(convmx)),
+        "fp64_relres_verified": float(rr64),
+        "mixed_relres_verified": float(rrmx),
+        "mixed_err_vs_fp64_solution": float(err_mx),
+        "spmv": {"fp32_gbs": gbs32, "fp64_gbs": gbs64, "fp32_ms": ms32, "fp64_ms": ms64,
+                 "copy_gbs": copy_gbs, "fp32_frac_of_copy": gbs32 / copy_gbs,
+                 "fp64_frac_of_copy": gbs64 / copy_gbs},
+        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_spmv<float,float,8,EPI_AXY,0>",
+                     "algorithmic_bytes": b32},
+        "device": dev,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(row, col, val, b, nv, args.solver, it64, eigs,
+                                            budget_s=args.cpu_budget)
+        line["cpu_baseline"]["gpu_speedup_fp64"] = line["cpu_baseline"]["value"] / mean_64
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,81 @@
+/*
+ * Extensions of the XAMG C API for the MI355X build.  These entry points have no
+ * counterpart in the reference's api/c/xamg_c_api.h; they expose what a GPU caller
+ * (and the parity tests and benchmark of this repository) needs beyond the
+ * reference's interface: error text, workload generators, direct access to the
+ * SpMV / dot kernels on C-API objects, per-RHS residual records and timing helpers.
+ * All return 0 on success unless stated otherwise.
+ */
+#ifndef XAMG_EXT_H_
+#define XAMG_EXT_H_
+
+#include "xamg_c_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call (empty string if none). */
+const char *xamgx_last_error(void);
+
+/* Device name / CU count / HBM size of the GPU in use, as text. */
+int xamgx_device_info(char *buf, int len);
+
+/* ---- workload generators (host) ---- */
+/* 3D 7-point Dirichlet Laplacian: diag 6, -1 per existing neighbour, lexicographic
+ * order i + nx*(j + ny*k), ascending columns.  nnz = 7n - 2(nx ny + ny nz + nx nz). */
+int64_t xamgx_gen_poisson7_nnz(int nx, int ny, int nz);
+int xamgx_gen_poisson7(int nx, int ny, int nz, int *row, int *col, double *val);
+/* 3D 27-point finite-volume anisotropic diffusion, coefficients k=(kx,ky,kz) times a
+ * per-cell log-normal(0,1) multiplier (harmonic face average), Dirichlet walls. */
+int64_t xamgx_gen_aniso27_nnz(int nx, int ny, int nz);
+int xamgx_gen_aniso27(int nx, int ny, int nz, double kx, double ky, double kz, uint32_t seed,
+                      int *row, int *col, double *val);
+/* 2D 5-point variable-coefficient diffusion on an s x s grid, log-normal(0,1) cell
+ * coefficients (harmonic face average), Dirichlet walls. */
+int64_t xamgx_gen_var5_nnz(int s);
+int xamgx_gen_var5(int s, uint32_t seed, int *row, int *col, double *val);
+/* b[i*nv+l] = 2*(pseudo_rand(k) - 0.5), k = (u32)(seed*2^27 + i*nv + l) (rand.h:33-73) */
+int xamgx_fill_rhs(double *b, int64_t n, int nv, uint32_t seed);
+
+/* ---- kernels on C-API objects ---- */
+/* y = A x (mode 0, csr_Ax_y) or y += A x (mode 1, csr_Axpy) on host arrays of nrows*nv
+ * elements; prec 0: FP64 values and vectors, 1: FP32 values and vectors,
+ * 2: BF16 values with FP32 vectors.  x/y are double* for prec 0, float* otherwise. */
+int xamgx_spmv_host(XAMG_matrix_id mid, int prec, int mode, int nv, const void *x, void *y);
+/* res[l] = sum_i x[i,l]*y[i,l] on two device vectors (blas.inl:521-589) */
+int xamgx_dot(XAMG_vector_id x, XAMG_vector_id y, int nv, double *res);
+
+/* ---- measurement ---- */
+/* Average device time (ms) of one y = A x launch over `iters` back-to-back launches
+ * after 10 warm-ups, timed with HIP events on the launch stream. */
+int xamgx_bench_spmv(XAMG_matrix_id mid, int prec, int nv, int iters, double *avg_ms);
+/* Sustained copy bandwidth (GB/s, read+write bytes) of a streaming copy kernel over
+ * `bytes` bytes, best of `iters` launches. */
+int xamgx_bench_copy(uint64_t bytes, int iters, double *gbs);
+/* Device time (ms) of the last solve (first kernel to the convergence readback). */
+int xamgx_solver_last_ms(XAMG_solver_id sid, double *ms);
+
+/* ---- solver records ---- */
+/* Residual history of the top solver: rows of nv squared residual norms, -1 for an
+ * RHS already converged (print.inl:119-142 table without the sqrt). */
+int xamgx_solver_get_history(XAMG_solver_id sid, double *out, int cap_rows, int *nrows);
+/* Squared initial / final residual norms per RHS (convergence.h res0/res). */
+int xamgx_solver_get_residuals(XAMG_solver_id sid, double *res0, double *res);
+
+/* ---- configuration ---- */
+/* Batch of iterations between host polls of the device halt flag, HIP-graph replay
+ * (0/1), grid sizes of streaming / reduction kernels, XCD-aware row mapping (0/1).
+ * A negative value keeps the current setting. */
+int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int grid_red,
+                     int xcd_remap);
+/* Spectrum bounds of D^{-1/2} A D^{-1/2} used by Chebyshev (the reference takes
+ * them from hypre's setup, hypre_base.inl:369-375). */
+int xamgx_matrix_set_eigs(XAMG_matrix_id mid, double lo, double hi);
+int xamgx_matrix_get_eigs(XAMG_matrix_id mid, double *lo, double *hi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XAMG_EXT_H_ */

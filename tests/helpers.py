@@ -1,0 +1,76 @@
+"""Shared test helpers: independent constructions of the test systems."""
+import numpy as np
+import scipy.sparse as sp
+
+
+def poisson7_scipy(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+
+    def T(n):
+        return sp.diags([-np.ones(n - 1), np.zeros(n), -np.ones(n - 1)], [-1, 0, 1])
+
+    def I(n):
+        return sp.identity(n)
+
+    A = (sp.kron(I(nz), sp.kron(I(ny), T(nx))) + sp.kron(I(nz), sp.kron(T(ny), I(nx)))
+         + sp.kron(T(nz), sp.kron(I(ny), I(nx))) + 6 * sp.identity(nx * ny * nz))
+    A = A.tocsr()
+    A.sort_indices()
+    return A
+
+
+def poisson_eigs(n):
+    """Extreme eigenvalues of D^{-1/2} A D^{-1/2} for the n^3 7-point Dirichlet Laplacian
+    with diagonal 6: 1 -+ cos(pi/(n+1))."""
+    c = np.cos(np.pi / (n + 1))
+    return 1.0 - c, 1.0 + c
+
+
+def paus_system(n):
+    """Reference examples/test system (generate.hpp:188-286 + mtx_gen.cpp:9-153):
+    2D Poisson on (n+1)^2 points with identity boundary rows, b = cos(h i) sin(h j)
+    (times 2 h^2 in the interior), h = pi/(2n)."""
+    h = np.pi / n / 2
+    N = (n + 1) * (n + 1)
+    rows, cols, vals = [], [], []
+    for i in range(n + 1):
+        for j in range(n + 1):
+            r = i * (n + 1) + j
+            if i == 0 or j == 0 or i == n or j == n:
+                rows.append(r); cols.append(r); vals.append(1.0)
+            else:
+                for (cc, v) in ((r - (n + 1), -1.0 / h / h), (r - 1, -1.0 / h / h),
+                                (r, 4.0 / h / h), (r + 1, -1.0 / h / h),
+                                (r + (n + 1), -1.0 / h / h)):
+                    rows.append(r); cols.append(cc); vals.append(v)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    A.sort_indices()
+    b = np.empty(N)
+    for k in range(N):
+        i, j = k // (n + 1), k % (n + 1)
+        v = np.cos(h * i) * np.sin(h * j)
+        if not (i == 0 or j == 0 or i == n or j == n):
+            v = v * 2.0 * h * h
+        b[k] = v
+    return A, b
+
+
+def read_paus(path):
+    d = open(path, "rb").read()
+    n, nnz = np.frombuffer(d[:8], dtype=np.uint32)
+    off = 8
+    row = np.frombuffer(d[off:off + 4 * (n + 1)], dtype=np.uint32); off += 4 * (n + 1)
+    col = np.frombuffer(d[off:off + 4 * nnz], dtype=np.uint32); off += 4 * nnz
+    val = np.frombuffer(d[off:off + 8 * nnz], dtype=np.float64); off += 8 * nnz
+    b = np.frombuffer(d[off:off + 8 * n], dtype=np.float64); off += 8 * n
+    x = np.frombuffer(d[off:off + 8 * n], dtype=np.float64)
+    return int(n), int(nnz), row, col, val, b, x
+
+
+def bf16_round(a):
+    """Round float64 values to bfloat16 (via float32, nearest-even) and back to float32."""
+    f = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    lsb = (f >> 16) & 1
+    f = ((f + 0x7FFF + lsb) >> 16) << 16
+    return f.astype(np.uint32).view(np.float32)

@@ -1,0 +1,150 @@
+"""Solver parity of the HIP path against the oracle (bicgstab.inl, pbicgstab.inl,
+jacobi.inl, chebyshev.inl, hsgs.inl, ir.inl, convergence.h).
+
+Bar: iteration counts within +-2 of the oracle, residual histories matching while the
+residual is well above round-off, the final residual verified in FP64, and the error
+against an FP64 direct solution reported.  Dot products are reduced in a different
+(fixed) order on the GPU, so histories agree to rounding, not bit for bit."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+import oracle as O
+from helpers import poisson7_scipy, poisson_eigs
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(method, prec=None, tol=1e-10, max_iters=500, extra=None):
+    p = {"solver": {"method": method, "rel_tolerance": tol, "abs_tolerance": 0.0,
+                    "max_iters": max_iters}}
+    if prec is not None:
+        p["preconditioner"] = dict(prec)
+    for k, v in (extra or {}).items():
+        p.setdefault(k, {}).update(v)
+    return p
+
+
+def _compare(gpu, orc, nv, hist_rtol=1e-6):
+    assert abs(gpu["iters"] - orc["iters"]) <= 2, (gpu["iters"], orc["iters"])
+    assert gpu["converged"] == orc["converged"]
+    hg, ho = gpu["hist"], orc["hist"]
+    k = min(len(hg), len(ho))
+    assert k >= 2
+    # compare while the residual is above 1e-7 of the initial one
+    for it in range(k):
+        for l in range(nv):
+            if ho[it, l] < 0 or hg[it, l] < 0:
+                continue
+            if ho[it, l] < 1e-14 * ho[0, l]:
+                continue
+            rel = abs(hg[it, l] - ho[it, l]) / ho[it, l]
+            tol = hist_rtol * max(1.0, (ho[0, l] / ho[it, l]) ** 0.5 * 1e-6)
+            assert rel <= max(tol, 1e-3 if it > 0.6 * k else tol), (it, l, hg[it, l], ho[it, l])
+
+
+CASES = [
+    ("BiCGStab", None),
+    ("PBiCGStab", {"method": "Jacobi"}),
+    ("PBiCGStab", {"method": "Chebyshev", "polynomial_order": 4}),
+    ("PBiCGStab", {"method": "Chebyshev", "polynomial_order": 2}),
+    ("PBiCGStab", {"method": "HSGS", "gs_blocks": 16}),
+]
+
+
+@pytest.mark.parametrize("method,prec", CASES)
+@pytest.mark.parametrize("nv", [1, 4])
+def test_krylov_parity_16cube(xamg, method, prec, nv):
+    nx = 16
+    row, col, val = xamg.poisson7(nx)
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 1)
+    params = _params(method, prec)
+    eigs = poisson_eigs(nx)
+    gpu = xamg.solve_system(row, col, val, b, nv, params, eigs=eigs)
+    orc = O.solve(params, O.csr(row, col, val), b, nv, eigs=eigs, gs_blocks=16)
+    _compare(gpu, orc, nv)
+    # FP64 check of the final residual and error against a direct solve
+    A = poisson7_scipy(nx)
+    X = gpu["x"].reshape(n, nv)
+    B = b.reshape(n, nv)
+    R = B - A @ X
+    for l in range(nv):
+        assert np.linalg.norm(R[:, l]) <= 1.01e-10 * np.linalg.norm(B[:, l]) * 1.5
+    xs = spl.spsolve(A.tocsc(), B)
+    assert np.linalg.norm(X - xs) / np.linalg.norm(xs) < 1e-8
+
+
+def test_cfg1_jacobi_32cube(xamg):
+    """32^3 7-point, nv=1, PBiCGStab + Jacobi (2 sweeps, 0.99) to 1e-8, seed 1."""
+    nx, nv = 32, 1
+    row, col, val = xamg.poisson7(nx)
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 1)
+    params = _params("PBiCGStab", {"method": "Jacobi"}, tol=1e-8)
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    orc = O.solve(params, O.csr(row, col, val), b, nv)
+    _compare(gpu, orc, nv)
+    assert gpu["converged"] == [True]
+
+
+def test_solver_deterministic(xamg):
+    nx, nv = 24, 4
+    row, col, val = xamg.poisson7(nx)
+    b = xamg.rhs(nx ** 3, nv, 8)
+    params = _params("PBiCGStab", {"method": "Jacobi"})
+    a = xamg.solve_system(row, col, val, b, nv, params)
+    c = xamg.solve_system(row, col, val, b, nv, params)
+    assert np.array_equal(a["hist"], c["hist"])
+    assert np.array_equal(a["x"], c["x"])
+
+
+def test_max_iters_exit_and_flags(xamg):
+    nx, nv = 16, 2
+    row, col, val = xamg.poisson7(nx)
+    b = xamg.rhs(nx ** 3, nv, 3)
+    params = _params("BiCGStab", tol=1e-14, max_iters=5)
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    orc = O.solve(params, O.csr(row, col, val), b, nv)
+    assert gpu["iters"] == 5 == orc["iters"]
+    assert gpu["converged"] == [False, False]
+
+
+def test_zero_rhs_converges_initially(xamg):
+    nx, nv = 8, 1
+    row, col, val = xamg.poisson7(nx)
+    b = np.zeros(nx ** 3)
+    params = _params("BiCGStab", tol=1e-10)
+    params["solver"]["abs_tolerance"] = 1e-12
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    assert gpu["iters"] == 0
+    assert gpu["converged"] == [True]
+
+
+@pytest.mark.parametrize("inner", ["BiCGStab", "PBiCGStab"])
+def test_ir_fp32_inner_fp64_outer(xamg, inner):
+    """IterativeRefinement with an FP32 inner solver reaches 1e-10 verified in FP64."""
+    nx, nv = 24, 4
+    row, col, val = xamg.poisson7(nx)
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 2)
+    params = {
+        "meta_solver": {"method": "IterativeRefinement", "max_iters": 20,
+                        "subsolver_reduced_precision": True, "rel_tolerance": 1e-10,
+                        "abs_tolerance": 0.0, "convergence_details": "check-L2"},
+        "solver": {"method": inner, "rel_tolerance": 1e-5, "abs_tolerance": 0.0,
+                   "max_iters": 200},
+    }
+    if inner == "PBiCGStab":
+        params["preconditioner"] = {"method": "Jacobi"}
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    assert all(gpu["converged"])
+    A = poisson7_scipy(nx)
+    X = gpu["x"].reshape(n, nv)
+    B = b.reshape(n, nv)
+    for l in range(nv):
+        assert np.linalg.norm(B[:, l] - A @ X[:, l]) <= 1e-10 * np.linalg.norm(B[:, l])
+    orc = O.solve(params, O.csr(row, col, val), b, nv)
+    assert abs(gpu["iters"] - orc["iters"]) <= 2
+    xs = spl.spsolve(A.tocsc(), B)
+    assert np.linalg.norm(X - xs) / np.linalg.norm(xs) < 1e-8

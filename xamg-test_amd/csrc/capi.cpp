@@ -1,0 +1,862 @@
+// XAMG C API (include/xamg_c_api.h) and its MI355X extensions (include/xamg_ext.h).
+// Handle tables follow api/c/xamg_c_api.cpp:64-85: indices into process-wide vectors,
+// never reused; *_destroy empties a slot.
+#include "../../include/xamg_c_api.h"
+#include "../../include/xamg_ext.h"
+
+#include "params.hpp"
+#include "runtime.hpp"
+#include "solvers.hpp"
+
+#include <sys/time.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace xamg {
+void fill_rhs(double *b, int64_t n, int nv, uint32_t seed);
+int64_t poisson7_nnz(int nx, int ny, int nz);
+void poisson7(int nx, int ny, int nz, int *row, int *col, double *val);
+int64_t aniso27_nnz(int nx, int ny, int nz);
+void aniso27(int nx, int ny, int nz, double kx, double ky, double kz, uint32_t seed, int *row,
+             int *col, double *val);
+int64_t var5_nnz(int s);
+void var5(int s, uint32_t seed, int *row, int *col, double *val);
+} // namespace xamg
+
+using namespace xamg;
+
+namespace {
+
+std::string g_err;
+
+struct Part {
+    int64_t size = 0;
+};
+
+struct Vec {
+    int mode = XAMG_MEM_LOCAL;
+    int64_t size = 0;
+    int nv = 0;
+    uint64_t ext_offset = 0;
+    std::vector<double> host; // LOCAL vectors
+    DBuf dev;                 // SHARED / DISTRIBUTED vectors (HBM)
+    bool on_device() const { return mode != XAMG_MEM_LOCAL; }
+};
+
+struct HostCsr {
+    int64_t nrows = 0, ncols = 0, nnz = 0;
+    int64_t row_offset = 0, col_offset = 0;
+    std::vector<uint32_t> row, col;
+    std::vector<double> val;
+};
+
+struct Mat {
+    std::unique_ptr<DevMatrix> m;
+    int part = -1;
+};
+
+struct Sol {
+    std::unique_ptr<SolverBase> s;
+    Solver<double> *top = nullptr;
+    GlobalParams params;
+    int mid = -1;
+    int x = -1, b = -1;
+    int nv = 0;
+    double last_ms = 0;
+};
+
+std::vector<GlobalParams *> g_params;
+std::vector<std::map<std::string, std::string> *> g_ovl;
+std::vector<Part *> g_parts;
+std::vector<Vec *> g_vecs;
+std::vector<HostCsr *> g_csr;
+std::vector<Mat *> g_mats;
+std::vector<Sol *> g_sols;
+
+template <typename T>
+T *slot(std::vector<T *> &v, size_t id, const char *what) {
+    if (id >= v.size() || v[id] == nullptr)
+        throw std::runtime_error(std::string("xamg: invalid ") + what + " handle");
+    return v[id];
+}
+
+template <typename F>
+int guard(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        fprintf(stderr, "%s\n", e.what());
+        return -1;
+    }
+}
+
+int default_nv() {
+    const char *e = getenv("XAMG_NV");
+    return e ? atoi(e) : 1;
+}
+
+bool print_enabled() {
+    const char *e = getenv("XAMG_LOG");
+    if (!e)
+        return true;
+    std::string s(e);
+    return !(s.find("^CONVERGENCE") != std::string::npos || s == "NONE" || s == "none" ||
+             s == "quiet");
+}
+
+void check_nv(int nv) {
+    XAMG_REQUIRE(nv == 1 || nv == 2 || nv == 4 || nv == 8 || nv == 16,
+                 "nv must be one of 1, 2, 4, 8, 16");
+}
+
+// print.inl:119-142 residual table, from the device history after the solve.
+void print_history(Sol *S, const SolveStats &st) {
+    const std::string role = S->params.has("meta_solver") ? "meta_solver" : "solver";
+    const std::string prefix = role == "meta_solver" ? "META" : "SOLV";
+    const ParamList &p = S->params.get(role);
+    if (!p.has("convergence_details") ||
+        p.get_string("convergence_details").find("print") == std::string::npos ||
+        p.get_string("convergence_details").find("noprint") != std::string::npos)
+        return;
+    if (!print_enabled())
+        return;
+    const int nv = S->nv;
+    for (int k = 0; k < st.nhist; k++) {
+        printf("%5s | %6d | ", prefix.c_str(), k);
+        for (int l = 0; l < nv; l++) {
+            double r = st.hist[(size_t)k * nv + l];
+            if (r < 0)
+                printf("%12s | %12s | ", "------------", "------------");
+            else
+                printf("%12.6e | %12.6e | ", std::sqrt(r), std::sqrt(r / st.res0[l]));
+        }
+        printf("\n");
+    }
+    fflush(stdout);
+}
+
+void do_solve(Sol *S, Vec *x, Vec *b, int nv) {
+    check_nv(nv);
+    XAMG_REQUIRE(nv == S->nv, "solve nv differs from the nv the solver was created with");
+    XAMG_REQUIRE(x->on_device() && b->on_device(), "solver vectors must be distributed");
+    XAMG_REQUIRE(x->nv == nv && b->nv == nv, "vector nv mismatch");
+    Runtime &rt = Runtime::get();
+    hipEvent_t e0, e1;
+    XAMG_HIP_CHECK(hipEventCreate(&e0));
+    XAMG_HIP_CHECK(hipEventCreate(&e1));
+    XAMG_HIP_CHECK(hipEventRecord(e0, rt.stream));
+    S->top->apply(x->dev.as<double>(), false, b->dev.as<double>(), nullptr);
+    XAMG_HIP_CHECK(hipEventRecord(e1, rt.stream));
+    XAMG_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    S->last_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SolveStats st;
+    S->top->stats(st, true);
+    print_history(S, st);
+}
+
+} // namespace
+
+extern "C" {
+
+int XAMG_init(int argc, char **argv, char *conf) {
+    (void)argc;
+    (void)argv;
+    (void)conf;
+    return guard([&] { Runtime::get(); });
+}
+
+int XAMG_finalize() {
+    return guard([&] {
+        if (Runtime::get().ready)
+            Runtime::get().sync();
+        for (auto &s : g_sols) {
+            delete s;
+            s = nullptr;
+        }
+        for (auto &p : g_params) {
+            delete p;
+            p = nullptr;
+        }
+        for (auto &o : g_ovl) {
+            delete o;
+            o = nullptr;
+        }
+        for (auto &p : g_parts) {
+            delete p;
+            p = nullptr;
+        }
+        for (auto &v : g_vecs) {
+            delete v;
+            v = nullptr;
+        }
+        for (auto &c : g_csr) {
+            delete c;
+            c = nullptr;
+        }
+        for (auto &m : g_mats) {
+            delete m;
+            m = nullptr;
+        }
+    });
+}
+
+int XAMG_get_nv() { return default_nv(); }
+int XAMG_id_get_proc() { return 0; }
+int XAMG_id_get_nprocs() { return 1; }
+void *XAMG_id_get_comm_ptr() { return nullptr; }
+
+// ---- parameters (xamg_c_api.cpp:150-300) ----
+int XAMG_param_create(XAMG_param_id *pid) {
+    return guard([&] {
+        g_params.push_back(new GlobalParams());
+        *pid = g_params.size() - 1;
+    });
+}
+
+static std::string num_str(double v) {
+    char b[64];
+    snprintf(b, sizeof(b), "%.17g", v);
+    return b;
+}
+
+static int add_value(char *role, char *key, const std::string &v, XAMG_param_id pid) {
+    return guard([&] { slot(g_params, pid, "param")->get(role).set(key, v); });
+}
+static int change_value(char *role, char *key, const std::string &v, XAMG_param_id pid, bool forced) {
+    return guard([&] {
+        GlobalParams *p = slot(g_params, pid, "param");
+        XAMG_REQUIRE(p->has(role), std::string("no parameter block for role ") + role);
+        ParamList &l = p->get(role);
+        // change_value requires an existing key; forced_change_value creates it
+        XAMG_REQUIRE(forced || l.has(key),
+                     std::string("parameter '") + key + "' is not set for role " + role);
+        l.set(key, v);
+    });
+}
+
+int XAMG_param_add_value_s(char *r, char *k, char *v, XAMG_param_id p) { return add_value(r, k, v, p); }
+int XAMG_param_add_value_i(char *r, char *k, uint32_t v, XAMG_param_id p) {
+    return add_value(r, k, std::to_string(v), p);
+}
+int XAMG_param_add_value_d(char *r, char *k, double v, XAMG_param_id p) { return add_value(r, k, num_str(v), p); }
+int XAMG_param_add_value_b(char *r, char *k, uint8_t v, XAMG_param_id p) {
+    return add_value(r, k, v ? "1" : "0", p);
+}
+int XAMG_param_change_value_s(char *r, char *k, char *v, XAMG_param_id p) { return change_value(r, k, v, p, false); }
+int XAMG_param_change_value_i(char *r, char *k, uint32_t v, XAMG_param_id p) {
+    return change_value(r, k, std::to_string(v), p, false);
+}
+int XAMG_param_change_value_d(char *r, char *k, double v, XAMG_param_id p) {
+    return change_value(r, k, num_str(v), p, false);
+}
+int XAMG_param_change_value_b(char *r, char *k, uint8_t v, XAMG_param_id p) {
+    return change_value(r, k, v ? "1" : "0", p, false);
+}
+int XAMG_param_forced_change_value_s(char *r, char *k, char *v, XAMG_param_id p) {
+    return change_value(r, k, v, p, true);
+}
+int XAMG_param_forced_change_value_i(char *r, char *k, uint32_t v, XAMG_param_id p) {
+    return change_value(r, k, std::to_string(v), p, true);
+}
+int XAMG_param_forced_change_value_d(char *r, char *k, double v, XAMG_param_id p) {
+    return change_value(r, k, num_str(v), p, true);
+}
+int XAMG_param_forced_change_value_b(char *r, char *k, uint8_t v, XAMG_param_id p) {
+    return change_value(r, k, v ? "1" : "0", p, true);
+}
+
+int XAMG_param_override_create(XAMG_param_override_id *ov) {
+    return guard([&] {
+        g_ovl.push_back(new std::map<std::string, std::string>());
+        *ov = g_ovl.size() - 1;
+    });
+}
+int XAMG_param_override_add_value(char *key, char *val, XAMG_param_override_id ov) {
+    return guard([&] { slot(g_ovl, ov, "override")->insert({key, val}); });
+}
+int XAMG_param_override_apply(char *role, XAMG_param_id pid, XAMG_param_override_id ov) {
+    return guard([&] {
+        GlobalParams *p = slot(g_params, pid, "param");
+        for (auto &kv : *slot(g_ovl, ov, "override"))
+            p->overrides[role].push_back(parse_override(kv.first, kv.second));
+    });
+}
+int XAMG_param_override_destroy(XAMG_param_override_id ov) {
+    return guard([&] {
+        if (ov < g_ovl.size()) {
+            delete g_ovl[ov];
+            g_ovl[ov] = nullptr;
+        }
+    });
+}
+void XAMG_param_set_default(XAMG_param_id pid) {
+    guard([&] { slot(g_params, pid, "param")->set_defaults(); });
+}
+int XAMG_param_print(XAMG_param_id pid) {
+    return guard([&] {
+        if (print_enabled())
+            slot(g_params, pid, "param")->print(stdout);
+    });
+}
+int XAMG_param_destroy(XAMG_param_id pid) {
+    return guard([&] {
+        if (pid < g_params.size()) {
+            delete g_params[pid];
+            g_params[pid] = nullptr;
+        }
+    });
+}
+
+// ---- partition ----
+int XAMG_part_create(XAMG_part_id *pid) {
+    return guard([&] {
+        g_parts.push_back(new Part());
+        *pid = g_parts.size() - 1;
+    });
+}
+int XAMG_part_construct(int size, XAMG_part_id pid) {
+    return guard([&] { slot(g_parts, pid, "part")->size = size; });
+}
+int XAMG_part_get_numa_size(XAMG_part_id pid) {
+    int r = 0;
+    if (guard([&] { r = (int)slot(g_parts, pid, "part")->size; }) != 0)
+        return -1;
+    return r;
+}
+int XAMG_part_destroy(XAMG_part_id pid) {
+    return guard([&] {
+        if (pid < g_parts.size()) {
+            delete g_parts[pid];
+            g_parts[pid] = nullptr;
+        }
+    });
+}
+
+// ---- vectors ----
+int XAMG_vector_create(XAMG_vector_id *vid, vector_alloc_mode mode) {
+    return guard([&] {
+        Vec *v = new Vec();
+        v->mode = mode;
+        g_vecs.push_back(v);
+        *vid = g_vecs.size() - 1;
+    });
+}
+int XAMG_vector_offset(uint64_t off, XAMG_vector_id vid) {
+    return guard([&] { slot(g_vecs, vid, "vector")->ext_offset = off; });
+}
+int XAMG_vector_alloc_d(int size, uint16_t nv, XAMG_vector_id vid) {
+    return guard([&] {
+        check_nv(nv);
+        Vec *v = slot(g_vecs, vid, "vector");
+        XAMG_REQUIRE(size >= 0, "negative vector size");
+        v->size = size;
+        v->nv = nv;
+        if (v->on_device()) {
+            v->dev.alloc(sizeof(double) * std::max<int64_t>(size, 1) * nv);
+            XAMG_HIP_CHECK(hipMemset(v->dev.p, 0, sizeof(double) * std::max<int64_t>(size, 1) * nv));
+        } else {
+            v->host.assign((size_t)size * nv, 0.0);
+        }
+    });
+}
+int XAMG_vector_set_part(XAMG_part_id pid, XAMG_vector_id vid) {
+    return guard([&] {
+        slot(g_parts, pid, "part");
+        slot(g_vecs, vid, "vector");
+    });
+}
+int XAMG_vector_set_val_d(double val, XAMG_vector_id vid) {
+    return guard([&] {
+        Vec *v = slot(g_vecs, vid, "vector");
+        if (v->on_device())
+            fill<double>(v->dev.as<double>(), v->size, v->nv, val);
+        else
+            std::fill(v->host.begin(), v->host.end(), val);
+        if (v->on_device())
+            Runtime::get().sync();
+    });
+}
+int XAMG_vector_download_d(double *ptr, uint64_t bs, uint64_t bo, uint16_t nv, XAMG_vector_id vid) {
+    return guard([&] {
+        Vec *v = slot(g_vecs, vid, "vector");
+        XAMG_REQUIRE(nv == v->nv, "download nv mismatch");
+        XAMG_REQUIRE(bo + bs <= (uint64_t)v->size, "download range outside the vector");
+        if (v->on_device()) {
+            Runtime::get().sync();
+            XAMG_HIP_CHECK(hipMemcpy(ptr, v->dev.as<double>() + bo * nv, sizeof(double) * bs * nv,
+                                     hipMemcpyDeviceToHost));
+        } else {
+            std::memcpy(ptr, v->host.data() + bo * nv, sizeof(double) * bs * nv);
+        }
+    });
+}
+int XAMG_vector_upload_v(XAMG_vector_id in, XAMG_vector_id out) {
+    return guard([&] {
+        Vec *a = slot(g_vecs, in, "vector"), *b = slot(g_vecs, out, "vector");
+        XAMG_REQUIRE(a->nv == b->nv, "upload nv mismatch");
+        XAMG_REQUIRE(a->ext_offset + a->size <= (uint64_t)b->size, "upload range outside the vector");
+        size_t bytes = sizeof(double) * a->size * a->nv;
+        double *dst = b->on_device() ? b->dev.as<double>() + a->ext_offset * a->nv
+                                     : b->host.data() + a->ext_offset * a->nv;
+        if (a->on_device() && b->on_device())
+            XAMG_HIP_CHECK(hipMemcpy(dst, a->dev.p, bytes, hipMemcpyDeviceToDevice));
+        else if (!a->on_device() && b->on_device())
+            XAMG_HIP_CHECK(hipMemcpy(dst, a->host.data(), bytes, hipMemcpyHostToDevice));
+        else if (a->on_device())
+            XAMG_HIP_CHECK(hipMemcpy(dst, a->dev.p, bytes, hipMemcpyDeviceToHost));
+        else
+            std::memcpy(dst, a->host.data(), bytes);
+    });
+}
+int XAMG_vector_upload_d(double *ptr, uint64_t bs, uint64_t bo, uint16_t nv, XAMG_vector_id vid) {
+    return guard([&] {
+        Vec *v = slot(g_vecs, vid, "vector");
+        XAMG_REQUIRE(nv == v->nv, "upload nv mismatch");
+        XAMG_REQUIRE(bo + bs <= (uint64_t)v->size, "upload range outside the vector");
+        if (v->on_device()) {
+            Runtime::get().sync();
+            XAMG_HIP_CHECK(hipMemcpy(v->dev.as<double>() + bo * nv, ptr, sizeof(double) * bs * nv,
+                                     hipMemcpyHostToDevice));
+        } else {
+            std::memcpy(v->host.data() + bo * nv, ptr, sizeof(double) * bs * nv);
+        }
+    });
+}
+int XAMG_vector_destroy(XAMG_vector_id vid) {
+    return guard([&] {
+        if (vid < g_vecs.size()) {
+            delete g_vecs[vid];
+            g_vecs[vid] = nullptr;
+        }
+    });
+}
+
+// ---- host CSR ----
+int XAMG_csr_matrix_create(XAMG_csr_matrix_id *id) {
+    return guard([&] {
+        g_csr.push_back(new HostCsr());
+        *id = g_csr.size() - 1;
+    });
+}
+int XAMG_csr_matrix_alloc(uint64_t nrows, uint64_t ncols, uint64_t nnz, XAMG_matrix_id id) {
+    return guard([&] {
+        HostCsr *c = slot(g_csr, id, "csr matrix");
+        XAMG_REQUIRE(nnz < (1ull << 32), "nnz exceeds 32-bit row pointers");
+        c->nrows = nrows;
+        c->ncols = ncols;
+        c->nnz = nnz;
+        c->row.assign(nrows + 1, 0);
+        c->col.assign(nnz, 0);
+        c->val.assign(nnz, 0.0);
+    });
+}
+int XAMG_csr_matrix_fill(int *row, int *col, double *val, XAMG_matrix_id id) {
+    return guard([&] {
+        HostCsr *c = slot(g_csr, id, "csr matrix");
+        for (int64_t i = 0; i <= c->nrows; i++)
+            c->row[i] = (uint32_t)row[i];
+        for (int64_t j = 0; j < c->nnz; j++) {
+            c->col[j] = (uint32_t)col[j];
+            c->val[j] = val[j];
+        }
+    });
+}
+int XAMG_csr_matrix_offset(int ro, int co, XAMG_matrix_id id) {
+    return guard([&] {
+        HostCsr *c = slot(g_csr, id, "csr matrix");
+        c->row_offset = ro;
+        c->col_offset = co;
+    });
+}
+uint32_t XAMG_csr_matrix_get_nrows(XAMG_csr_matrix_id id) {
+    uint32_t r = 0;
+    guard([&] { r = (uint32_t)slot(g_csr, id, "csr matrix")->nrows; });
+    return r;
+}
+int XAMG_csr_matrix_destroy(XAMG_csr_matrix_id id) {
+    return guard([&] {
+        if (id < g_csr.size()) {
+            delete g_csr[id];
+            g_csr[id] = nullptr;
+        }
+    });
+}
+
+// ---- device matrix ----
+int XAMG_matrix_create(XAMG_matrix_id *mid) {
+    return guard([&] {
+        g_mats.push_back(new Mat());
+        *mid = g_mats.size() - 1;
+    });
+}
+int XAMG_matrix_set_part(XAMG_part_id pid, XAMG_matrix_id mid) {
+    return guard([&] {
+        slot(g_parts, pid, "part");
+        slot(g_mats, mid, "matrix")->part = (int)pid;
+    });
+}
+int XAMG_matrix_construct(XAMG_csr_matrix_id cid, XAMG_matrix_id mid) {
+    return guard([&] {
+        HostCsr *c = slot(g_csr, cid, "csr matrix");
+        Mat *m = slot(g_mats, mid, "matrix");
+        XAMG_REQUIRE(c->row_offset == 0 && c->nrows == c->ncols,
+                     "one GPU holds the whole square matrix: expected row offset 0 and nrows == ncols");
+        XAMG_REQUIRE(c->nrows == 0 || c->row[c->nrows] == (uint32_t)c->nnz,
+                     "row pointer end differs from nnz");
+        for (int64_t j = 0; j < c->nnz; j++)
+            XAMG_REQUIRE((int64_t)c->col[j] < c->ncols, "column index out of range");
+        for (int64_t i = 0; i < c->nrows; i++)
+            XAMG_REQUIRE(c->row[i] <= c->row[i + 1], "row pointers must be non-decreasing");
+        auto dm = std::make_unique<DevMatrix>();
+        dm->n = c->nrows;
+        dm->ncols = c->ncols;
+        dm->nnz = c->nnz;
+        dm->hrow = c->row;
+        dm->hcol = c->col;
+        dm->hval = c->val;
+        dm->upload();
+        m->m = std::move(dm);
+    });
+}
+int XAMG_matrix_destroy(XAMG_matrix_id mid) {
+    return guard([&] {
+        if (mid < g_mats.size()) {
+            delete g_mats[mid];
+            g_mats[mid] = nullptr;
+        }
+    });
+}
+
+// ---- solvers ----
+static int create_solver(XAMG_matrix_id mid, XAMG_param_id pid, int x, int b, XAMG_solver_id *sid) {
+    return guard([&] {
+        Mat *m = slot(g_mats, mid, "matrix");
+        XAMG_REQUIRE(m->m != nullptr, "matrix not constructed");
+        GlobalParams *p = slot(g_params, pid, "param");
+        Sol *S = new Sol();
+        S->params = *p;
+        S->mid = (int)mid;
+        S->x = x;
+        S->b = b;
+        S->nv = x >= 0 ? slot(g_vecs, x, "vector")->nv : default_nv();
+        check_nv(S->nv);
+        try {
+            S->s = construct_solver_hierarchy(S->params, m->m.get(), S->nv, &S->top);
+        } catch (...) {
+            delete S;
+            throw;
+        }
+        g_sols.push_back(S);
+        *sid = g_sols.size() - 1;
+    });
+}
+
+int XAMG_solver_create1(XAMG_matrix_id mid, XAMG_param_id pid, XAMG_solver_id *sid) {
+    return create_solver(mid, pid, -1, -1, sid);
+}
+int XAMG_solver_create2(XAMG_matrix_id mid, XAMG_param_id pid, XAMG_vector_id x, XAMG_vector_id b,
+                        XAMG_solver_id *sid) {
+    return create_solver(mid, pid, (int)x, (int)b, sid);
+}
+int XAMG_solver_renew_params(XAMG_param_id pid, XAMG_solver_id sid) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        GlobalParams *p = slot(g_params, pid, "param");
+        for (auto &r : p->roles)
+            for (auto &kv : r.second.kv)
+                S->params.get(r.first).set(kv.first, kv.second);
+        S->s->renew(S->params);
+    });
+}
+int XAMG_solver_solve1(uint16_t nv, XAMG_solver_id sid) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        XAMG_REQUIRE(S->x >= 0 && S->b >= 0, "solve1 needs a solver created with vectors");
+        do_solve(S, slot(g_vecs, S->x, "vector"), slot(g_vecs, S->b, "vector"), nv);
+    });
+}
+int XAMG_solver_solve2(uint16_t nv, XAMG_vector_id x, XAMG_vector_id b, XAMG_solver_id sid) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        do_solve(S, slot(g_vecs, x, "vector"), slot(g_vecs, b, "vector"), nv);
+    });
+}
+int XAMG_solver_print_info(XAMG_solver_id sid) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        DevMatrix *A = slot(g_mats, S->mid, "matrix")->m.get();
+        if (print_enabled())
+            printf("A: nrows %lld, nonzeros %lld, avg nnz/row %.3f\n", (long long)A->n,
+                   (long long)A->nnz, A->n ? (double)A->nnz / A->n : 0.0);
+    });
+}
+int XAMG_solver_get_convergence_info(int *conv, uint16_t nv, XAMG_solver_id sid) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        XAMG_REQUIRE(nv <= S->nv, "nv larger than the solver's");
+        SolveStats st;
+        S->top->stats(st, false);
+        for (int l = 0; l < nv; l++)
+            conv[l] = st.converged[l];
+    });
+}
+int XAMG_solver_get_iter_info(int *info, XAMG_solver_id sid) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        SolveStats st;
+        S->top->stats(st, false);
+        info[0] = st.iters;
+        info[1] = st.sub_iters;
+    });
+}
+int XAMG_solver_destroy(XAMG_solver_id sid) {
+    return guard([&] {
+        if (sid < g_sols.size()) {
+            delete g_sols[sid];
+            g_sols[sid] = nullptr;
+        }
+    });
+}
+
+double XAMG_timer() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return (double)tv.tv_sec + 1e-6 * (double)tv.tv_usec;
+}
+
+int XAMG_barrier() {
+    return guard([&] {
+        if (Runtime::get().ready)
+            Runtime::get().sync();
+    });
+}
+
+// ---------------------------------------------------------------------------
+// extensions
+// ---------------------------------------------------------------------------
+const char *xamgx_last_error(void) { return g_err.c_str(); }
+
+int xamgx_device_info(char *buf, int len) {
+    return guard([&] {
+        Runtime &rt = Runtime::get();
+        hipDeviceProp_t p;
+        XAMG_HIP_CHECK(hipGetDeviceProperties(&p, rt.device));
+        snprintf(buf, len, "%s | %s | CUs %d | HBM %.1f GiB", p.name, p.gcnArchName,
+                 p.multiProcessorCount, p.totalGlobalMem / 1073741824.0);
+    });
+}
+
+int64_t xamgx_gen_poisson7_nnz(int nx, int ny, int nz) { return poisson7_nnz(nx, ny, nz); }
+int xamgx_gen_poisson7(int nx, int ny, int nz, int *row, int *col, double *val) {
+    return guard([&] { poisson7(nx, ny, nz, row, col, val); });
+}
+int64_t xamgx_gen_aniso27_nnz(int nx, int ny, int nz) { return aniso27_nnz(nx, ny, nz); }
+int xamgx_gen_aniso27(int nx, int ny, int nz, double kx, double ky, double kz, uint32_t seed,
+                      int *row, int *col, double *val) {
+    return guard([&] { aniso27(nx, ny, nz, kx, ky, kz, seed, row, col, val); });
+}
+int64_t xamgx_gen_var5_nnz(int s) { return var5_nnz(s); }
+int xamgx_gen_var5(int s, uint32_t seed, int *row, int *col, double *val) {
+    return guard([&] { var5(s, seed, row, col, val); });
+}
+int xamgx_fill_rhs(double *b, int64_t n, int nv, uint32_t seed) {
+    return guard([&] { fill_rhs(b, n, nv, seed); });
+}
+
+int xamgx_spmv_host(XAMG_matrix_id mid, int prec, int mode, int nv, const void *x, void *y) {
+    return guard([&] {
+        check_nv(nv);
+        DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
+        XAMG_REQUIRE(A != nullptr, "matrix not constructed");
+        XAMG_REQUIRE(prec >= 0 && prec <= 2, "prec must be 0, 1 or 2");
+        XAMG_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (Ax_y) or 1 (Axpy)");
+        size_t es = prec == 0 ? 8 : 4;
+        size_t bx = es * std::max<int64_t>(A->ncols, 1) * nv, by = es * std::max<int64_t>(A->n, 1) * nv;
+        DBuf dx(bx), dy(by);
+        XAMG_HIP_CHECK(hipMemcpy(dx.p, x, es * A->ncols * nv, hipMemcpyHostToDevice));
+        XAMG_HIP_CHECK(hipMemcpy(dy.p, y, es * A->n * nv, hipMemcpyHostToDevice));
+        const int epi = mode == 0 ? EPI_AXY : EPI_AXPY;
+        if (prec == 0) {
+            SpmvArgs<double> a{};
+            a.x = dx.as<double>();
+            a.y = dy.as<double>();
+            spmv<double>(*A, Prec::F64, epi, 0, nv, a, red_args(), FinArgs<double>{});
+        } else {
+            SpmvArgs<float> a{};
+            a.x = dx.as<float>();
+            a.y = dy.as<float>();
+            spmv<float>(*A, prec == 1 ? Prec::F32 : Prec::BF16, epi, 0, nv, a, red_args(),
+                        FinArgs<float>{});
+        }
+        Runtime::get().sync();
+        XAMG_HIP_CHECK(hipMemcpy(y, dy.p, es * A->n * nv, hipMemcpyDeviceToHost));
+    });
+}
+
+int xamgx_dot(XAMG_vector_id x, XAMG_vector_id y, int nv, double *res) {
+    return guard([&] {
+        Vec *a = slot(g_vecs, x, "vector"), *b = slot(g_vecs, y, "vector");
+        XAMG_REQUIRE(a->on_device() && b->on_device(), "dot needs device vectors");
+        XAMG_REQUIRE(a->nv == nv && b->nv == nv && a->size == b->size, "dot shape mismatch");
+        // z = 1*x (scratch copy) with the fused dot(z, y): the reduction path of the solvers
+        DBuf z(sizeof(double) * std::max<int64_t>(a->size, 1) * nv);
+        EwArgs<double> e;
+        e.n = a->size;
+        e.z = z.as<double>();
+        e.x = a->dev.as<double>();
+        e.y2 = z.as<double>();
+        e.w = b->dev.as<double>();
+        ew<double>(EW_COPY3, 1, nv, e, red_args(), FinArgs<double>{});
+        Runtime::get().sync();
+        XAMG_HIP_CHECK(hipMemcpy(res, Runtime::get().ws.result, sizeof(double) * nv,
+                                 hipMemcpyDeviceToHost));
+    });
+}
+
+int xamgx_bench_spmv(XAMG_matrix_id mid, int prec, int nv, int iters, double *avg_ms) {
+    return guard([&] {
+        check_nv(nv);
+        DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
+        XAMG_REQUIRE(A != nullptr, "matrix not constructed");
+        Runtime &rt = Runtime::get();
+        size_t es = prec == 0 ? 8 : 4;
+        DBuf dx(es * std::max<int64_t>(A->n, 1) * nv), dy(es * std::max<int64_t>(A->n, 1) * nv);
+        auto launch = [&]() {
+            if (prec == 0) {
+                SpmvArgs<double> a{};
+                a.x = dx.as<double>();
+                a.y = dy.as<double>();
+                spmv<double>(*A, Prec::F64, EPI_AXY, 0, nv, a, red_args(), FinArgs<double>{});
+            } else {
+                SpmvArgs<float> a{};
+                a.x = dx.as<float>();
+                a.y = dy.as<float>();
+                spmv<float>(*A, prec == 1 ? Prec::F32 : Prec::BF16, EPI_AXY, 0, nv, a, red_args(),
+                            FinArgs<float>{});
+            }
+        };
+        if (prec == 0)
+            fill<double>(dx.as<double>(), A->n, nv, 1.0);
+        else
+            fill<float>(dx.as<float>(), A->n, nv, 1.0f);
+        for (int i = 0; i < 10; i++)
+            launch();
+        hipEvent_t e0, e1;
+        XAMG_HIP_CHECK(hipEventCreate(&e0));
+        XAMG_HIP_CHECK(hipEventCreate(&e1));
+        XAMG_HIP_CHECK(hipEventRecord(e0, rt.stream));
+        for (int i = 0; i < iters; i++)
+            launch();
+        XAMG_HIP_CHECK(hipEventRecord(e1, rt.stream));
+        XAMG_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / iters;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+int xamgx_solver_last_ms(XAMG_solver_id sid, double *ms) {
+    return guard([&] { *ms = slot(g_sols, sid, "solver")->last_ms; });
+}
+
+int xamgx_solver_get_history(XAMG_solver_id sid, double *out, int cap_rows, int *nrows) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        SolveStats st;
+        S->top->stats(st, true);
+        int n = std::min(st.nhist, cap_rows);
+        if (n > 0)
+            std::memcpy(out, st.hist.data(), sizeof(double) * n * S->nv);
+        *nrows = st.nhist;
+    });
+}
+
+int xamgx_solver_get_residuals(XAMG_solver_id sid, double *res0, double *res) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        SolveStats st;
+        S->top->stats(st, false);
+        for (int l = 0; l < S->nv; l++) {
+            res0[l] = st.res0[l];
+            res[l] = st.res[l];
+        }
+    });
+}
+
+int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int grid_red,
+                     int xcd_remap) {
+    return guard([&] {
+        if (batch >= 0)
+            engine_cfg().batch = std::max(1, batch);
+        if (use_graphs >= 0)
+            engine_cfg().use_graphs = use_graphs;
+        LaunchCfg &c = launch_cfg();
+        if (grid_spmv > 0)
+            c.grid_spmv = grid_spmv;
+        if (grid_ew > 0)
+            c.grid_ew = grid_ew;
+        if (grid_red > 0)
+            c.grid_red = std::min(grid_red, 4096);
+        if (xcd_remap >= 0)
+            c.xcd_remap = xcd_remap;
+    });
+}
+
+int xamgx_matrix_set_eigs(XAMG_matrix_id mid, double lo, double hi) {
+    return guard([&] {
+        DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
+        XAMG_REQUIRE(A != nullptr, "matrix not constructed");
+        A->min_eig = lo;
+        A->max_eig = hi;
+    });
+}
+int xamgx_matrix_get_eigs(XAMG_matrix_id mid, double *lo, double *hi) {
+    return guard([&] {
+        DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
+        XAMG_REQUIRE(A != nullptr, "matrix not constructed");
+        if (std::isnan(A->min_eig) || std::isnan(A->max_eig))
+            A->estimate_eigs();
+        *lo = A->min_eig;
+        *hi = A->max_eig;
+    });
+}
+
+} // extern "C"
+
+extern "C" int xamgx_bench_copy(uint64_t bytes, int iters, double *gbs) {
+    return guard([&] {
+        Runtime &rt = Runtime::get();
+        bytes = (bytes / 16) * 16;
+        DBuf a(bytes), b(bytes);
+        XAMG_HIP_CHECK(hipMemset(a.p, 1, bytes));
+        hipEvent_t e0, e1;
+        XAMG_HIP_CHECK(hipEventCreate(&e0));
+        XAMG_HIP_CHECK(hipEventCreate(&e1));
+        double best = 0;
+        launch_copy(a.p, b.p, (int64_t)bytes, rt.stream);
+        for (int i = 0; i < iters; i++) {
+            XAMG_HIP_CHECK(hipEventRecord(e0, rt.stream));
+            launch_copy(a.p, b.p, (int64_t)bytes, rt.stream);
+            XAMG_HIP_CHECK(hipEventRecord(e1, rt.stream));
+            XAMG_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::max(best, 2.0 * (double)bytes / (ms * 1e-3) / 1e9);
+        }
+        *gbs = best;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}

@@ -1,0 +1,78 @@
+// Shared host/device definitions for the MI355X XAMG solve path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+namespace xamg {
+
+constexpr int kMaxNV = 16;
+constexpr int kBlock = 256;    // threads per workgroup for streaming kernels
+constexpr int kNumXcd = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kMaxDots = 3;    // fused reductions per kernel (phi, psi, theta)
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string &s) : std::runtime_error(s) {}
+};
+
+#define XAMG_HIP_CHECK(expr)                                                                      \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            char b_[512];                                                                          \
+            snprintf(b_, sizeof(b_), "HIP error %s at %s:%d: %s", hipGetErrorName(e_), __FILE__,   \
+                     __LINE__, #expr);                                                             \
+            throw ::xamg::HipError(b_);                                                            \
+        }                                                                                          \
+    } while (0)
+
+#define XAMG_REQUIRE(cond, msg)                                                                    \
+    do {                                                                                           \
+        if (!(cond))                                                                               \
+            throw std::runtime_error(std::string("xamg: ") + (msg));                              \
+    } while (0)
+
+enum class Prec : int { F64 = 0, F32 = 1, BF16 = 2 };
+
+inline size_t prec_size(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
+
+// Device-resident convergence record of one solver instance (convergence.h:49-303).
+// Lives in HBM; kernels update it, the host reads it back only at batch ends.
+struct ConvState {
+    double res0[kMaxNV];
+    double res[kMaxNV];
+    double tol[kMaxNV];
+    int status[kMaxNV];
+    int iter;      // iterations counted like stats.increment_iters_counter
+    int sub_iter;  // inner iterations (IterativeRefinement)
+    int done;      // all RHS converged at the last check
+    int halt;      // loop kernels return immediately once set
+    int nhist;     // residual-history rows written
+    int hist_cap;
+    int pad[2];
+};
+
+// Krylov scalars of one BiCGStab/PBiCGStab instance, precision of the iteration.
+template <typename F>
+struct KrylovScalars {
+    F rho[kMaxNV], alpha[kMaxNV], nalpha[kMaxNV], omega[kMaxNV], nomega[kMaxNV];
+    F beta[kMaxNV], nbo[kMaxNV], ac[kMaxNV], oc[kMaxNV], iconv[kMaxNV];
+    F s1[kMaxNV];
+};
+
+// Reduction workspace shared by every fused-dot kernel on a stream: per-block
+// partial sums (pair-major) plus a ticket counter for the last-block final pass.
+struct ReduceWs {
+    double *partials;  // [kMaxDots*kMaxNV][max_blocks]
+    unsigned *counter; // zero between kernels (the last block resets it)
+    double *result;    // [kMaxDots*kMaxNV], written by the last block
+    int max_blocks;
+};
+
+} // namespace xamg

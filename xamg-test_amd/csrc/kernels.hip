@@ -1,0 +1,691 @@
+// HIP kernels of the XAMG solve path for gfx950 (CDNA4, wave64).
+//
+// All kernels are bandwidth-bound streaming kernels; none uses MFMA.  Floating-point
+// contraction is disabled at build time (-ffp-contract=off) so every element-wise
+// result follows the reference's operation order bit for bit; only the reduction
+// order of dot products differs (fixed tree instead of a sequential sum).
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace xamg {
+
+LaunchCfg &launch_cfg() {
+    static LaunchCfg cfg;
+    return cfg;
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+template <typename T, typename V>
+__device__ __forceinline__ T to_t(V v) {
+    return (T)v;
+}
+template <>
+__device__ __forceinline__ float to_t<float, __hip_bfloat16>(__hip_bfloat16 v) {
+    return __bfloat162float(v);
+}
+template <>
+__device__ __forceinline__ double to_t<double, __hip_bfloat16>(__hip_bfloat16 v) {
+    return (double)__bfloat162float(v);
+}
+
+// XCD-aware workgroup -> contiguous-range map (see kernels.hpp).
+__device__ __forceinline__ int logical_block(int b, int G, int remap) {
+    if (remap && (G % kNumXcd) == 0)
+        return (b % kNumXcd) * (G / kNumXcd) + b / kNumXcd;
+    return b;
+}
+
+// Device-side convergence check (convergence.h:209-262): per-RHS residual update,
+// history row, comparison in the iteration precision F, mask update.
+template <typename F>
+__device__ void conv_check(const FinArgs<F> &f, const F *rn, int nv, int initial) {
+    ConvState *cs = f.cs;
+    if (!f.conv_check || cs == nullptr)
+        return;
+    if (initial) {
+        for (int l = 0; l < nv; l++) {
+            cs->res0[l] = (double)rn[l];
+            cs->tol[l] = fmax(f.abs_tol, f.rel_tol * sqrt(cs->res0[l]));
+        }
+    }
+    for (int l = 0; l < nv; l++)
+        if (!cs->status[l])
+            cs->res[l] = (double)rn[l];
+    if (f.hist != nullptr && cs->nhist < cs->hist_cap) {
+        for (int l = 0; l < nv; l++)
+            f.hist[(int64_t)cs->nhist * nv + l] = cs->status[l] ? -1.0 : cs->res[l];
+        cs->nhist++;
+    }
+    int flag = 1;
+    for (int l = 0; l < nv; l++) {
+        if (cs->status[l])
+            continue;
+        F ar = (F)sqrt(cs->res[l]);
+        if (ar > (F)cs->tol[l])
+            flag = 0;
+        else
+            cs->status[l] = 1;
+    }
+    if (flag)
+        cs->done = 1;
+    else if (f.iconv != nullptr)
+        for (int l = 0; l < nv; l++)
+            if (cs->status[l])
+                f.iconv[l] = (F)0.0;
+}
+
+// Scalar updates of the Krylov loops, executed by one thread of the last workgroup.
+template <typename F>
+__device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
+    KrylovScalars<F> *ks = f.ks;
+    ConvState *cs = f.cs;
+    F v[kMaxNV];
+    switch (f.fin) {
+    case FIN_INIT:
+        for (int l = 0; l < nv; l++) {
+            v[l] = (F)sums[l];
+            ks->rho[l] = v[l];
+        }
+        if (f.conv_check) {
+            conv_check(f, v, nv, 1);
+            cs->halt = cs->done;
+        }
+        break;
+    case FIN_ALPHA:
+        cs->iter++;
+        for (int l = 0; l < nv; l++) {
+            F s1 = (F)sums[l];
+            F al = ks->rho[l] / s1;
+            ks->alpha[l] = al;
+            ks->nalpha[l] = al * (F)-1.0;
+        }
+        break;
+    case FIN_OMEGA:
+        for (int l = 0; l < nv; l++) {
+            F s2 = (F)sums[l], s3 = (F)sums[nv + l], s4 = (F)sums[2 * nv + l];
+            F om = s2 / s3;
+            ks->omega[l] = om;
+            ks->nomega[l] = om * (F)-1.0;
+            ks->ac[l] = ks->iconv[l] * ks->alpha[l];
+            ks->oc[l] = ks->iconv[l] * om;
+            F res = om * s2;
+            res = (F)1.0 * s4 + (F)(-1.0) * res;
+            v[l] = res;
+        }
+        conv_check(f, v, nv, 0);
+        break;
+    case FIN_BETA:
+        for (int l = 0; l < nv; l++) {
+            F s1 = (F)sums[l];
+            F s2 = s1 / ks->rho[l];
+            F s3 = ks->alpha[l] / ks->omega[l];
+            F be = s2 * s3;
+            ks->beta[l] = be;
+            ks->rho[l] = s1;
+            F t = ks->omega[l] * be;
+            ks->nbo[l] = t * (F)-1.0;
+        }
+        break;
+    case FIN_NORM_INIT:
+        for (int l = 0; l < nv; l++)
+            v[l] = (F)sums[l];
+        if (f.conv_check) {
+            conv_check(f, v, nv, 1);
+            cs->halt = cs->done;
+        }
+        break;
+    case FIN_NORM:
+    case FIN_NORM_NOINC:
+        if (f.fin == FIN_NORM)
+            cs->iter++;
+        for (int l = 0; l < nv; l++)
+            v[l] = (F)sums[l];
+        if (f.conv_check) {
+            conv_check(f, v, nv, 0);
+            cs->halt = cs->done;
+        }
+        break;
+    default:
+        break;
+    }
+}
+
+// Block reduction of ND per-RHS accumulators, store of the block partial, and the
+// last-block final pass.  Threads with equal (tid % NV) share an RHS index.
+template <typename T, int NV, int ND>
+__device__ void reduce_and_finish(double (&acc)[ND], const RedArgs &r, const FinArgs<T> &f) {
+    __shared__ double lds[kBlock / 64][ND][NV];
+    __shared__ unsigned ticket;
+    __shared__ double sums[ND * NV];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int G = gridDim.x;
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        double s = acc[k];
+#pragma unroll
+        for (int off = 32; off >= NV; off >>= 1)
+            s += __shfl_xor(s, off);
+        if (lane < NV)
+            lds[wave][k][lane] = s;
+    }
+    __syncthreads();
+    if (tid < ND * NV) {
+        const int k = tid / NV, l = tid % NV;
+        double s = lds[0][k][l];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++)
+            s += lds[w][k][l];
+        r.ws.partials[(int64_t)tid * r.ws.max_blocks + blockIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0)
+        ticket = __hip_atomic_fetch_add(r.ws.counter, 1u, __ATOMIC_ACQ_REL,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (ticket != (unsigned)(G - 1))
+        return;
+    __threadfence();
+    // Final pass: wave w sums pairs p = w, w+4, ... over all blocks in a fixed order.
+    for (int p = wave; p < ND * NV; p += kBlock / 64) {
+        const double *src = r.ws.partials + (int64_t)p * r.ws.max_blocks;
+        double s = 0.0;
+        for (int b = lane; b < G; b += 64)
+            s += __builtin_nontemporal_load(src + b);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1)
+            s += __shfl_xor(s, off);
+        if (lane == 0)
+            sums[p] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int p = 0; p < ND * NV; p++)
+            r.ws.result[p] = sums[p];
+        apply_fin(f, sums, NV);
+        __hip_atomic_store(r.ws.counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV with fused epilogue and optional fused dot products.
+// One lane per (row, RHS); NV lanes of a row read the same val/col entry and
+// NV consecutive x values (one 8*NV-byte segment per neighbour).
+// ---------------------------------------------------------------------------
+template <typename V, typename T, int NV, int EPI, int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_spmv(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
+    if (r.halt != nullptr && *r.halt)
+        return;
+    const V *__restrict__ val = (const V *)a.val;
+    const uint32_t *__restrict__ row = a.row;
+    const uint32_t *__restrict__ col = a.col;
+    const T *__restrict__ x = a.x;
+    constexpr int RPB = kBlock / NV;
+    const int tid = threadIdx.x, lr = tid / NV, l = tid % NV;
+    const int G = gridDim.x;
+    const int L = logical_block(blockIdx.x, G, remap);
+    const int64_t r0 = a.n * L / G, r1 = a.n * (L + 1) / G;
+    double dacc[ND > 0 ? ND : 1];
+#pragma unroll
+    for (int k = 0; k < (ND > 0 ? ND : 1); k++)
+        dacc[k] = 0.0;
+    for (int64_t base = r0; base < r1; base += RPB) {
+        const int64_t i = base + lr;
+        if (i < r1) {
+            const int64_t e = i * NV + l;
+            const uint32_t j0 = row[i], j1 = row[i + 1];
+            T acc = (EPI == EPI_AXPY) ? a.y[e] : (T)0;
+            for (uint32_t j = j0; j < j1; ++j) {
+                const uint32_t c = col[j];
+                T xv = x[(int64_t)c * NV + l];
+                if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST)
+                    xv = xv * a.dcol[c];
+                acc = acc + xv * to_t<T>(val[j]);
+            }
+            T out;
+            if constexpr (EPI == EPI_AXY || EPI == EPI_AXPY) {
+                out = acc;
+            } else if constexpr (EPI == EPI_RESID) {
+                out = (T)1.0 * a.b[e] + (T)(-1.0) * acc;
+            } else if constexpr (EPI == EPI_JACOBI) {
+                out = a.xin[e] + (a.wcoef[l] * a.d[i]) * (a.b[e] - acc);
+            } else {
+                const T di = a.d[i];
+                const T rr = a.b[e] * di;
+                T t2 = acc * di;
+                out = a.coef * rr + (T)1.0 * t2;
+                if constexpr (EPI == EPI_CHEB_LAST) {
+                    out = out * di;
+                    if (a.xin != nullptr)
+                        out = (T)1.0 * out + (T)1.0 * a.xin[e];
+                }
+            }
+            a.y[e] = out;
+            if (a.y2 != nullptr)
+                a.y2[e] = out;
+            if (a.y3 != nullptr)
+                a.y3[e] = out;
+            if constexpr (ND == 1) {
+                const double wv = a.w != nullptr ? (double)a.w[e] : (double)out;
+                dacc[0] += (double)out * wv;
+            } else if constexpr (ND == 3) {
+                const double wv = (double)a.w[e];
+                dacc[0] += (double)out * wv;
+                dacc[1] += (double)out * (double)out;
+                dacc[2] += wv * wv;
+            }
+        }
+    }
+    if constexpr (ND > 0)
+        reduce_and_finish<T, NV, ND>(dacc, r, f);
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise kernels (row ranges, one lane per (row, RHS) element).
+// ---------------------------------------------------------------------------
+template <typename T, int NV, int OP, int ND>
+__global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
+    if (r.halt != nullptr && *r.halt) {
+        return;
+    }
+    if (r.set_halt_if_done && r.cs != nullptr && r.cs->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            r.cs->halt = 1;
+        return;
+    }
+    const int tid = threadIdx.x;
+    const int l = tid % NV;
+    const int G = gridDim.x;
+    const int L = logical_block(blockIdx.x, G, remap);
+    const int64_t r0 = a.n * L / G, r1 = a.n * (L + 1) / G;
+    const int64_t e0 = r0 * NV, e1 = r1 * NV;
+    const T ca = a.a != nullptr ? a.a[l] : (T)1.0;
+    const T cb = a.b != nullptr ? a.b[l] : (T)1.0;
+    const T cc = a.c != nullptr ? a.c[l] : (T)1.0;
+    double dacc[ND > 0 ? ND : 1];
+#pragma unroll
+    for (int k = 0; k < (ND > 0 ? ND : 1); k++)
+        dacc[k] = 0.0;
+    for (int64_t e = e0 + tid; e < e1; e += kBlock) {
+        T out;
+        if constexpr (OP == EW_FILL) {
+            out = a.c0;
+        } else if constexpr (OP == EW_AXPBY_Z) {
+            out = ca * a.x[e] + cb * a.y[e];
+        } else if constexpr (OP == EW_AXPBYPCZ) {
+            out = ca * a.x[e] + cb * a.y[e] + cc * a.z[e];
+        } else if constexpr (OP == EW_JACOBI0) {
+            const int64_t i = e / NV;
+            out = (ca * a.d[i]) * a.x[e];
+        } else if constexpr (OP == EW_CHEB0 || OP == EW_CHEB0_LAST) {
+            const int64_t i = e / NV;
+            const T di = a.d[i];
+            out = a.c0 * (a.x[e] * di);
+            if constexpr (OP == EW_CHEB0_LAST) {
+                out = out * di;
+                if (a.y != nullptr)
+                    out = (T)1.0 * out + (T)1.0 * a.y[e];
+            }
+        } else if constexpr (OP == EW_COPY3) {
+            out = (T)1.0 * a.x[e];
+            a.y2[e] = out;
+            if (a.y3 != nullptr)
+                a.y3[e] = out;
+        } else if constexpr (OP == EW_ADD_CONV) {
+            using O = typename std::conditional<std::is_same<T, double>::value, float, double>::type;
+            out = (T)1.0 * (T)((const O *)a.xo)[e] + (T)1.0 * a.z[e];
+        } else if constexpr (OP == EW_CONVERT) {
+            using O = typename std::conditional<std::is_same<T, double>::value, float, double>::type;
+            out = (T)((const O *)a.xo)[e];
+        } else if constexpr (OP == EW_SCAL_D) {
+            out = a.z[e] * a.d[e / NV];
+        } else { // EW_JAC_UPD
+            out = (ca * cc) * (a.y[e] * a.d[e / NV]) + (T)1.0 * a.z[e];
+        }
+        a.z[e] = out;
+        if constexpr (ND == 1) {
+            const double wv = a.w != nullptr ? (double)a.w[e] : (double)out;
+            dacc[0] += (double)out * wv;
+        }
+    }
+    if constexpr (ND > 0)
+        reduce_and_finish<T, NV, ND>(dacc, r, f);
+}
+
+// Dense GEMV for the coarse Direct solve (dense_matvec.inl:100-117): one lane per
+// (row, RHS) sums the columns in ascending order; A is stored column-major (A^T row-major)
+// so the lanes of a wave read consecutive rows of one column.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_dense_mv(const T *__restrict__ AT, int64_t n, const T *__restrict__ x, T *__restrict__ y,
+               int nv, const int *halt) {
+    if (halt != nullptr && *halt)
+        return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * nv)
+        return;
+    const int64_t i = e / nv;
+    const int l = (int)(e % nv);
+    T s = 0;
+    for (int64_t j = 0; j < n; j++)
+        s = s + AT[j * n + i] * x[j * nv + l];
+    y[e] = s;
+}
+
+// Block-hybrid SGS.  One workgroup per core block; the NV lanes of a row relax the
+// RHS in parallel while the rows of the block are visited in order (forward then
+// backward), exactly the sequential loop of csr_gs.inl:77-130 per block.
+template <typename V, typename T, int NV>
+__global__ void __launch_bounds__(64)
+    k_hsgs_offd(const uint32_t *__restrict__ row, const uint32_t *__restrict__ col,
+                const V *__restrict__ val, int64_t n, const T *__restrict__ b,
+                const T *__restrict__ x, T *__restrict__ t, int nblocks, int64_t bsize,
+                const int *halt) {
+    if (halt != nullptr && *halt)
+        return;
+    // t = 1*b + (-1)*(sum over off-block entries of x*a) for every row, grid-stride
+    const int64_t tot = n * NV;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / NV;
+        const int l = (int)(e % NV);
+        const int64_t blk = i / bsize;
+        const int64_t off = blk * bsize;
+        const int64_t end = off + bsize < n ? off + bsize : n;
+        T acc = 0;
+        for (uint32_t j = row[i]; j < row[i + 1]; j++) {
+            const int64_t c = col[j];
+            if (c >= off && c < end)
+                continue;
+            acc = acc + x[c * NV + l] * to_t<T>(val[j]);
+        }
+        t[e] = (T)1.0 * b[e] + (T)(-1.0) * acc;
+    }
+}
+
+template <typename V, typename T, int NV>
+__global__ void __launch_bounds__(64)
+    k_hsgs_diag(const uint32_t *__restrict__ row, const uint32_t *__restrict__ col,
+                const V *__restrict__ val, const T *__restrict__ inv_diag, int64_t n,
+                const T *__restrict__ t, T *__restrict__ x, const T *__restrict__ relax,
+                const T *__restrict__ iconv, int64_t bsize, const int *halt) {
+    if (halt != nullptr && *halt)
+        return;
+    const int64_t off = (int64_t)blockIdx.x * bsize;
+    if (off >= n)
+        return;
+    const int64_t len = off + bsize < n ? bsize : n - off;
+    const int l = threadIdx.x;
+    if (l >= NV)
+        return;
+    const T w = iconv != nullptr ? relax[l] * iconv[l] : relax[l];
+    for (int64_t ii = 0; ii < 2 * len; ii++) {
+        const int64_t i = ii < len ? off + ii : off + (2 * len - 1 - ii);
+        T sum = t[i * NV + l];
+        for (uint32_t j = row[i]; j < row[i + 1]; j++) {
+            const int64_t c = col[j];
+            if (c < off || c >= off + len)
+                continue;
+            sum -= x[c * NV + l] * to_t<T>(val[j]);
+        }
+        const T coeff = w * inv_diag[i];
+        x[i * NV + l] += coeff * sum;
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+template <typename V, typename T, int NV>
+static void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
+                    hipStream_t s) {
+    const LaunchCfg &c = launch_cfg();
+    int G = nd > 0 ? c.grid_red : c.grid_spmv;
+    if (nd > 0 && G > r.ws.max_blocks)
+        G = r.ws.max_blocks;
+    int64_t need = (a.n * NV + kBlock - 1) / kBlock;
+    if (need < G)
+        G = (int)(need > 0 ? need : 1);
+    dim3 g(G), b(kBlock);
+#define XAMG_SPMV_CASE(E, D)                                                                       \
+    if (epi == E && nd == D) {                                                                     \
+        hipLaunchKernelGGL((k_spmv<V, T, NV, E, D>), g, b, 0, s, a, r, f, c.xcd_remap);            \
+        return;                                                                                    \
+    }
+    XAMG_SPMV_CASE(EPI_AXY, 0)
+    XAMG_SPMV_CASE(EPI_AXY, 1)
+    XAMG_SPMV_CASE(EPI_AXY, 3)
+    XAMG_SPMV_CASE(EPI_AXPY, 0)
+    XAMG_SPMV_CASE(EPI_RESID, 0)
+    XAMG_SPMV_CASE(EPI_RESID, 1)
+    XAMG_SPMV_CASE(EPI_JACOBI, 0)
+    XAMG_SPMV_CASE(EPI_CHEB, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_LAST, 0)
+#undef XAMG_SPMV_CASE
+    throw std::runtime_error("xamg: unsupported spmv epilogue/dot combination");
+}
+
+template <typename V, typename T>
+void launch_spmv(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                 const FinArgs<T> &f, hipStream_t s) {
+    switch (nv) {
+    case 1:
+        spmv_nv<V, T, 1>(epi, ndots, a, r, f, s);
+        break;
+    case 2:
+        spmv_nv<V, T, 2>(epi, ndots, a, r, f, s);
+        break;
+    case 4:
+        spmv_nv<V, T, 4>(epi, ndots, a, r, f, s);
+        break;
+    case 8:
+        spmv_nv<V, T, 8>(epi, ndots, a, r, f, s);
+        break;
+    case 16:
+        spmv_nv<V, T, 16>(epi, ndots, a, r, f, s);
+        break;
+    default:
+        throw std::runtime_error("xamg: nv must be one of 1, 2, 4, 8, 16");
+    }
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, int NV>
+static void ew_nv(int op, int nd, const EwArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
+                  hipStream_t s) {
+    const LaunchCfg &c = launch_cfg();
+    int G = nd > 0 ? c.grid_red : c.grid_ew;
+    if (nd > 0 && G > r.ws.max_blocks)
+        G = r.ws.max_blocks;
+    int64_t need = (a.n * NV + kBlock - 1) / kBlock;
+    if (need < G)
+        G = (int)(need > 0 ? need : 1);
+    dim3 g(G), b(kBlock);
+#define XAMG_EW_CASE(O, D)                                                                         \
+    if (op == O && nd == D) {                                                                      \
+        hipLaunchKernelGGL((k_ew<T, NV, O, D>), g, b, 0, s, a, r, f, c.xcd_remap);                 \
+        return;                                                                                    \
+    }
+    XAMG_EW_CASE(EW_FILL, 0)
+    XAMG_EW_CASE(EW_AXPBY_Z, 0)
+    XAMG_EW_CASE(EW_AXPBY_Z, 1)
+    XAMG_EW_CASE(EW_AXPBYPCZ, 0)
+    XAMG_EW_CASE(EW_JACOBI0, 0)
+    XAMG_EW_CASE(EW_CHEB0, 0)
+    XAMG_EW_CASE(EW_CHEB0_LAST, 0)
+    XAMG_EW_CASE(EW_COPY3, 0)
+    XAMG_EW_CASE(EW_ADD_CONV, 0)
+    XAMG_EW_CASE(EW_CONVERT, 0)
+    XAMG_EW_CASE(EW_SCAL_D, 0)
+    XAMG_EW_CASE(EW_COPY3, 1)
+    XAMG_EW_CASE(EW_JAC_UPD, 0)
+#undef XAMG_EW_CASE
+    throw std::runtime_error("xamg: unsupported elementwise op/dot combination");
+}
+
+template <typename T>
+void launch_ew(int op, int ndots, int nv, const EwArgs<T> &a, const RedArgs &r,
+               const FinArgs<T> &f, hipStream_t s) {
+    switch (nv) {
+    case 1:
+        ew_nv<T, 1>(op, ndots, a, r, f, s);
+        break;
+    case 2:
+        ew_nv<T, 2>(op, ndots, a, r, f, s);
+        break;
+    case 4:
+        ew_nv<T, 4>(op, ndots, a, r, f, s);
+        break;
+    case 8:
+        ew_nv<T, 8>(op, ndots, a, r, f, s);
+        break;
+    case 16:
+        ew_nv<T, 16>(op, ndots, a, r, f, s);
+        break;
+    default:
+        throw std::runtime_error("xamg: nv must be one of 1, 2, 4, 8, 16");
+    }
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_dense_mv(const T *at, int64_t n, const T *x, T *y, int nv, const int *halt,
+                     hipStream_t s) {
+    if (n <= 0)
+        return;
+    int64_t tot = n * nv;
+    hipLaunchKernelGGL(k_dense_mv<T>, dim3((unsigned)((tot + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, s, at, n, x, y, nv, halt);
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename V, typename T, int NV>
+static void hsgs_nv(const uint32_t *row, const uint32_t *col, const V *val, const T *inv_diag,
+                    int64_t n, const T *b, T *x, T *t, const T *relax, const T *iconv,
+                    int nblocks, int64_t bsize, const int *halt, hipStream_t s) {
+    int64_t tot = n * NV;
+    int g1 = (int)std::min<int64_t>((tot + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_hsgs_offd<V, T, NV>), dim3(g1 > 0 ? g1 : 1), dim3(256), 0, s, row, col,
+                       val, n, b, x, t, nblocks, bsize, halt);
+    hipLaunchKernelGGL((k_hsgs_diag<V, T, NV>), dim3(nblocks), dim3(64), 0, s, row, col, val,
+                       inv_diag, n, t, x, relax, iconv, bsize, halt);
+}
+
+template <typename V, typename T>
+void launch_hsgs(const uint32_t *row, const uint32_t *col, const V *val, const T *inv_diag,
+                 int64_t n, int nv, const T *b, T *x, T *t, const T *relax, const T *iconv,
+                 int nblocks, const int *halt, hipStream_t s) {
+    if (n <= 0)
+        return;
+    // core block size from split_range (misc.h:47-56) with 8-byte elements
+    int64_t cs = n / nblocks;
+    cs += 8 - (cs % 8);
+    int nb = (int)((n + cs - 1) / cs);
+    switch (nv) {
+    case 1:
+        hsgs_nv<V, T, 1>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
+        break;
+    case 2:
+        hsgs_nv<V, T, 2>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
+        break;
+    case 4:
+        hsgs_nv<V, T, 4>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
+        break;
+    case 8:
+        hsgs_nv<V, T, 8>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
+        break;
+    case 16:
+        hsgs_nv<V, T, 16>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
+        break;
+    default:
+        throw std::runtime_error("xamg: nv must be one of 1, 2, 4, 8, 16");
+    }
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    if (cs != nullptr) {
+        for (int l = 0; l < kMaxNV; l++) {
+            cs->res0[l] = cs->res[l] = cs->tol[l] = 0.0;
+            cs->status[l] = 0;
+        }
+        cs->iter = cs->sub_iter = cs->done = cs->halt = cs->nhist = 0;
+        cs->hist_cap = hist_cap;
+    }
+    if (ks != nullptr)
+        for (int l = 0; l < kMaxNV; l++)
+            ks->iconv[l] = (T)1.0;
+    (void)nv;
+}
+
+template <typename T>
+void launch_state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_state_init<T>, dim3(1), dim3(64), 0, s, ks, cs, nv, hist_cap);
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+// explicit instantiations
+template void launch_spmv<double, double>(int, int, int, const SpmvArgs<double> &,
+                                          const RedArgs &, const FinArgs<double> &, hipStream_t);
+template void launch_spmv<float, float>(int, int, int, const SpmvArgs<float> &, const RedArgs &,
+                                        const FinArgs<float> &, hipStream_t);
+template void launch_spmv<__hip_bfloat16, float>(int, int, int, const SpmvArgs<float> &,
+                                                 const RedArgs &, const FinArgs<float> &,
+                                                 hipStream_t);
+template void launch_ew<double>(int, int, int, const EwArgs<double> &, const RedArgs &,
+                                const FinArgs<double> &, hipStream_t);
+template void launch_ew<float>(int, int, int, const EwArgs<float> &, const RedArgs &,
+                               const FinArgs<float> &, hipStream_t);
+template void launch_dense_mv<double>(const double *, int64_t, const double *, double *, int,
+                                      const int *, hipStream_t);
+template void launch_dense_mv<float>(const float *, int64_t, const float *, float *, int,
+                                     const int *, hipStream_t);
+template void launch_hsgs<double, double>(const uint32_t *, const uint32_t *, const double *,
+                                          const double *, int64_t, int, const double *, double *,
+                                          double *, const double *, const double *, int,
+                                          const int *, hipStream_t);
+template void launch_hsgs<float, float>(const uint32_t *, const uint32_t *, const float *,
+                                        const float *, int64_t, int, const float *, float *,
+                                        float *, const float *, const float *, int, const int *,
+                                        hipStream_t);
+template void launch_state_init<double>(KrylovScalars<double> *, ConvState *, int, int,
+                                        hipStream_t);
+template void launch_state_init<float>(KrylovScalars<float> *, ConvState *, int, int,
+                                       hipStream_t);
+
+} // namespace xamg
+
+namespace xamg {
+
+// Streaming copy (16-byte lanes) used to measure the attainable HBM bandwidth that the
+// SpMV roofline fraction is quoted against.
+__global__ void __launch_bounds__(kBlock) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                 int64_t n16) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s) {
+    const int64_t n16 = bytes / 16;
+    int grid = (int)std::min<int64_t>((n16 + kBlock - 1) / kBlock, 256 * 32);
+    hipLaunchKernelGGL(k_copy, dim3(grid > 0 ? grid : 1), dim3(kBlock), 0, s, (const uint4 *)src,
+                       (uint4 *)dst, n16);
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace xamg

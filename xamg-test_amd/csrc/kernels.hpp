@@ -1,0 +1,143 @@
+// Kernel launch interface of the MI355X XAMG solve path (host side).
+//
+// Every streaming kernel walks a contiguous row range per workgroup, with the
+// workgroup -> range map made XCD-aware (MI355X dispatches workgroups round-robin
+// over 8 XCDs, each with its own L2): the workgroups of one XCD cover one contiguous
+// eighth of the rows, so the gathered neighbours of a stencil stay in that XCD's L2.
+//
+// Reductions are deterministic: per-workgroup partials in fixed lane/wave order, then
+// the last workgroup to finish (ticket counter) sums the partials in workgroup order
+// and applies the solver's scalar update ("final op") on the device.
+#pragma once
+
+#include "common.hpp"
+
+namespace xamg {
+
+// SpMV epilogues.  acc = sum_j x[col_j] * a_ij in ascending j (csr_matvec.inl:213-231).
+enum Epi : int {
+    EPI_AXY = 0,      // y = acc                                     (csr_Ax_y)
+    EPI_AXPY = 1,     // y += acc                                    (csr_Axpy)
+    EPI_RESID = 2,    // y = 1*b + (-1)*acc                           (solver.h:183-190)
+    EPI_JACOBI = 3,   // y = xin + (w*d_i)*(b - acc)                  (blas_merged_jacobi.inl:80-85)
+    EPI_CHEB = 4,     // y = c*(b*d_i) + 1*(acc*d_i), x pre-scaled by d_col (chebyshev.inl:172-178)
+    EPI_CHEB_LAST = 5 // as EPI_CHEB then y = 1*(y*d_i) [+ 1*xold]   (chebyshev.inl:180-181)
+};
+
+// Elementwise ops on [n][nv] vectors with per-RHS coefficients (nullptr = 1).
+enum Ew : int {
+    EW_FILL = 0,       // z = c0
+    EW_AXPBY_Z = 1,    // z = a*x + b*y                               (blas_ext.inl:30-102)
+    EW_AXPBYPCZ = 2,   // z = a*x + b*y + c*z                         (blas_ext.inl:106-189)
+    EW_JACOBI0 = 3,    // z = (w*d_i)*b  (zero initial guess)          (blas_merged_jacobi.inl:66-72)
+    EW_CHEB0 = 4,      // z = c*(b*d_i)                               (chebyshev.inl:169-170)
+    EW_CHEB0_LAST = 5, // z = 1*((c*(b*d_i))*d_i) [+1*zold]           (order-1 polynomial)
+    EW_COPY3 = 6,      // z = 1*x, y2 = z, y3 = z (initial residual r0, r, p from b)
+    EW_ADD_CONV = 7,   // z(T) = 1*(T)x32 + 1*z                       (ir.inl:156-158)
+    EW_CONVERT = 8,    // z(T) = (T)x(other precision)                (convert.inl:31-83)
+    EW_SCAL_D = 9,     // z = z * d_i  (row scaling)                  (blas.inl:173-183)
+    EW_JAC_UPD = 10,   // z = (a*c)*(y*d_i) + 1*z  (check-mode Jacobi, jacobi.inl:80-84)
+};
+
+// Scalar update applied by the last workgroup after a fused reduction.
+enum Fin : int {
+    FIN_NONE = 0,   // only store the reduced sums
+    FIN_INIT = 1,   // rho = res0 = ||r0||^2, tolerance, initial check   (stats.h:195-206)
+    FIN_ALPHA = 2,  // iter++, alpha = rho/(v,r0)                          (pbicgstab.inl:114-131)
+    FIN_OMEGA = 3,  // omega, masked alpha/omega, res = theta - omega*phi, check (:146-171)
+    FIN_BETA = 4,   // beta, rho update, -(omega*beta)                     (:176-199)
+    FIN_NORM_INIT = 5, // res0 = ||r||^2, tolerance, check (Jacobi/MG/IR/Cheb check mode)
+    FIN_NORM = 6,      // iter++ then check ||r||^2 against the tolerance
+    FIN_NORM_NOINC = 7 // check ||r||^2 without counting an iteration
+};
+
+template <typename F>
+struct FinArgs {
+    KrylovScalars<F> *ks = nullptr;
+    ConvState *cs = nullptr;
+    double *hist = nullptr; // [hist_cap][nv]
+    int fin = FIN_NONE;
+    int conv_check = 0;
+    double abs_tol = 0, rel_tol = 0;
+    F *iconv = nullptr; // mask updated on partial convergence (nullptr: no mask)
+};
+
+struct RedArgs {
+    ReduceWs ws;
+    const int *halt = nullptr; // skip the launch's work once set (loop early exit)
+    int set_halt_if_done = 0;  // kernel sets *halt when the solver's done flag is up
+    ConvState *cs = nullptr;
+};
+
+template <typename T>
+struct SpmvArgs {
+    const uint32_t *row;
+    const uint32_t *col;
+    const void *val; // V-typed
+    int64_t n;
+    const T *x;
+    T *y;
+    // epilogue inputs
+    const T *b = nullptr;    // RESID/JACOBI/CHEB
+    const T *xin = nullptr;  // JACOBI previous iterate; CHEB_LAST previous x
+    const T *d = nullptr;    // row scaling (inv_diag / inv_sqrt_diag)
+    const T *dcol = nullptr; // CHEB: column pre-scaling of x
+    T coef = 0;              // CHEB coefficient c_q
+    const T *wcoef = nullptr; // JACOBI per-RHS relax factors
+    const T *w = nullptr;    // second operand of the fused dot(s)
+    T *y2 = nullptr, *y3 = nullptr; // extra copies of y (initial residual: r, p)
+};
+
+template <typename T>
+struct EwArgs {
+    int64_t n; // rows
+    T *z;
+    const T *x = nullptr, *y = nullptr;
+    const T *a = nullptr, *b = nullptr, *c = nullptr; // per-RHS coefficients, nullptr = 1
+    const T *d = nullptr;                             // per-row scaling
+    T c0 = 0;                                         // scalar (FILL, CHEB0 coefficient)
+    const void *xo = nullptr;                         // other-precision input (CONVERT/ADD_CONV)
+    const T *w = nullptr;                             // fused dot: dot(z, w)
+    T *y2 = nullptr, *y3 = nullptr;                   // COPY3 extra outputs
+};
+
+struct LaunchCfg {
+    int grid_spmv = 2048;
+    int grid_ew = 2048;
+    int grid_red = 1024;
+    int xcd_remap = 1;
+};
+
+LaunchCfg &launch_cfg();
+
+// V value type: double, float or __hip_bfloat16; T vector type: double or float.
+template <typename V, typename T>
+void launch_spmv(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                 const FinArgs<T> &f, hipStream_t s);
+
+template <typename T>
+void launch_ew(int op, int ndots, int nv, const EwArgs<T> &a, const RedArgs &r,
+               const FinArgs<T> &f, hipStream_t s);
+
+// Dense GEMV y = A x for the coarse-grid Direct solver (dense_matvec.inl:100-117);
+// `at` holds A column-major.
+template <typename T>
+void launch_dense_mv(const T *at, int64_t n, const T *x, T *y, int nv, const int *halt,
+                     hipStream_t s);
+
+// Block-hybrid symmetric Gauss-Seidel sweep (csr_gs.inl:32-134 + blas2.h:225-263):
+// nblocks contiguous row blocks; t = b - A_offd x_old, then forward and backward
+// relaxation inside each block.  xnew may alias nothing; x is read-only.
+template <typename V, typename T>
+void launch_hsgs(const uint32_t *row, const uint32_t *col, const V *val, const T *inv_diag,
+                 int64_t n, int nv, const T *b, T *x, T *t, const T *relax, const T *iconv,
+                 int nblocks, const int *halt, hipStream_t s);
+
+template <typename T>
+void launch_state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap, hipStream_t s);
+
+} // namespace xamg
+
+namespace xamg {
+void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
+}

@@ -1,0 +1,327 @@
+// Host runtime: stream, reduction workspace, device CSR copies (see runtime.hpp).
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace xamg {
+
+Runtime &Runtime::get() {
+    static Runtime rt;
+    if (!rt.ready)
+        rt.init();
+    return rt;
+}
+
+void Runtime::init() {
+    int ndev = 0;
+    XAMG_HIP_CHECK(hipGetDeviceCount(&ndev));
+    XAMG_REQUIRE(ndev > 0, "no HIP device visible");
+    XAMG_HIP_CHECK(hipGetDevice(&device));
+    // default-priority stream (no priority changes, see the box notes on stream priorities)
+    XAMG_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const int max_blocks = 4096;
+    partials.alloc(sizeof(double) * kMaxDots * kMaxNV * max_blocks);
+    counter.alloc(256);
+    result.alloc(sizeof(double) * kMaxDots * kMaxNV);
+    XAMG_HIP_CHECK(hipMemset(counter.p, 0, 256));
+    ws.partials = partials.as<double>();
+    ws.counter = counter.as<unsigned>();
+    ws.result = result.as<double>();
+    ws.max_blocks = max_blocks;
+    XAMG_HIP_CHECK(hipHostMalloc((void **)&pinned_flags, 4096, hipHostMallocDefault));
+    std::memset(pinned_flags, 0, 4096);
+    ready = true;
+}
+
+RedArgs red_args(const int *halt) {
+    RedArgs r;
+    r.ws = Runtime::get().ws;
+    r.halt = halt;
+    return r;
+}
+
+// --- bf16 conversion on the host: round to nearest even (IEEE binary32 -> bfloat16) ---
+static uint16_t f32_to_bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        return (uint16_t)((u >> 16) | 0x40); // quiet NaN
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+void DevMatrix::upload() {
+    Runtime &rt = Runtime::get();
+    row.alloc(sizeof(uint32_t) * (n + 1));
+    col.alloc(sizeof(uint32_t) * std::max<int64_t>(nnz, 1));
+    val64.alloc(sizeof(double) * std::max<int64_t>(nnz, 1));
+    XAMG_HIP_CHECK(hipMemcpy(row.p, hrow.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        XAMG_HIP_CHECK(hipMemcpy(col.p, hcol.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
+        XAMG_HIP_CHECK(hipMemcpy(val64.p, hval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    (void)rt;
+}
+
+const void *DevMatrix::values(Prec p) {
+    if (p == Prec::F64)
+        return val64.p;
+    if (p == Prec::F32) {
+        if (!val32.p) {
+            std::vector<float> v(std::max<int64_t>(nnz, 1));
+            for (int64_t j = 0; j < nnz; j++)
+                v[j] = (float)hval[j];
+            val32.alloc(sizeof(float) * v.size());
+            XAMG_HIP_CHECK(hipMemcpy(val32.p, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
+        }
+        return val32.p;
+    }
+    if (!valbf.p) {
+        std::vector<uint16_t> v(std::max<int64_t>(nnz, 1));
+        for (int64_t j = 0; j < nnz; j++)
+            v[j] = f32_to_bf16_bits((float)hval[j]);
+        valbf.alloc(sizeof(uint16_t) * v.size());
+        XAMG_HIP_CHECK(hipMemcpy(valbf.p, v.data(), sizeof(uint16_t) * v.size(), hipMemcpyHostToDevice));
+    }
+    return valbf.p;
+}
+
+// csr_matrix.h:323-389: 1.0/a_ii and sqrt(1.0/a_ii) computed in double from the values
+// of the precision in use, stored in that precision.
+template <typename T>
+static DBuf make_diag(const DevMatrix &A, bool sqrt_inv) {
+    std::vector<T> d(std::max<int64_t>(A.n, 1), (T)0);
+    for (int64_t i = 0; i < A.n; i++)
+        for (uint32_t j = A.hrow[i]; j < A.hrow[i + 1]; j++)
+            if (A.hcol[j] == (uint32_t)i) {
+                T a = (T)A.hval[j];
+                d[i] = sqrt_inv ? (T)std::sqrt(1.0 / a) : (T)(1.0 / a);
+            }
+    DBuf b(sizeof(T) * d.size());
+    XAMG_HIP_CHECK(hipMemcpy(b.p, d.data(), sizeof(T) * d.size(), hipMemcpyHostToDevice));
+    return b;
+}
+
+template <>
+const double *DevMatrix::inv_diag<double>() {
+    if (!inv_diag64.p)
+        inv_diag64 = make_diag<double>(*this, false);
+    return inv_diag64.as<double>();
+}
+template <>
+const float *DevMatrix::inv_diag<float>() {
+    if (!inv_diag32.p)
+        inv_diag32 = make_diag<float>(*this, false);
+    return inv_diag32.as<float>();
+}
+template <>
+const double *DevMatrix::inv_sqrt_diag<double>() {
+    if (!inv_sqrt_diag64.p)
+        inv_sqrt_diag64 = make_diag<double>(*this, true);
+    return inv_sqrt_diag64.as<double>();
+}
+template <>
+const float *DevMatrix::inv_sqrt_diag<float>() {
+    if (!inv_sqrt_diag32.p)
+        inv_sqrt_diag32 = make_diag<float>(*this, true);
+    return inv_sqrt_diag32.as<float>();
+}
+
+// inverse.inl:31-104 Gauss-Jordan without pivoting in precision T, stored column-major.
+template <typename T>
+static DBuf make_dense_inverse(const DevMatrix &A) {
+    const int64_t n = A.n, nc = 2 * n;
+    XAMG_REQUIRE(n <= 20000, "Direct: coarse matrix too large for a dense inverse");
+    std::vector<T> e((size_t)(n * nc), (T)0);
+    for (int64_t i = 0; i < n; i++) {
+        e[i * nc + i + n] = (T)1.0;
+        for (uint32_t j = A.hrow[i]; j < A.hrow[i + 1]; j++)
+            e[i * nc + A.hcol[j]] = (T)A.hval[j];
+    }
+    for (int64_t i = 0; i < n; i++) {
+        T *ri = &e[i * nc];
+        for (int64_t k = nc - 1; k > i; --k)
+            ri[k] /= ri[i];
+        ri[i] = (T)1.0;
+        for (int64_t j = i + 1; j < n; j++) {
+            T *rj = &e[j * nc];
+            for (int64_t k = nc - 1; k > i; --k)
+                rj[k] -= rj[i] * ri[k];
+            rj[i] = (T)0.0;
+        }
+    }
+    for (int64_t i = 0; i < n; i++) {
+        int64_t r = n - 1 - i;
+        T *ri = &e[r * nc];
+        for (int64_t j = 0; j < r; j++) {
+            T *rj = &e[j * nc];
+            for (int64_t k = nc / 2; k < nc; k++)
+                rj[k] -= rj[r] * ri[k];
+            rj[r] = (T)0.0;
+        }
+    }
+    std::vector<T> at((size_t)std::max<int64_t>(n * n, 1));
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++)
+            at[j * n + i] = e[i * nc + n + j];
+    DBuf b(sizeof(T) * at.size());
+    XAMG_HIP_CHECK(hipMemcpy(b.p, at.data(), sizeof(T) * at.size(), hipMemcpyHostToDevice));
+    return b;
+}
+
+template <>
+const double *DevMatrix::dense_inverse<double>() {
+    if (!dense_inv64.p)
+        dense_inv64 = make_dense_inverse<double>(*this);
+    return dense_inv64.as<double>();
+}
+template <>
+const float *DevMatrix::dense_inverse<float>() {
+    if (!dense_inv32.p)
+        dense_inv32 = make_dense_inverse<float>(*this);
+    return dense_inv32.as<float>();
+}
+
+// Lanczos on S = D^{-1/2} A D^{-1/2} (symmetric for symmetric A); the extreme Ritz
+// values of the tridiagonal matrix bound the spectrum used by Chebyshev.
+void DevMatrix::estimate_eigs(int steps) {
+    const int64_t N = n;
+    if (N == 0)
+        return;
+    std::vector<double> d(N, 1.0);
+    for (int64_t i = 0; i < N; i++)
+        for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
+            if (hcol[j] == (uint32_t)i)
+                d[i] = std::sqrt(1.0 / hval[j]);
+    steps = (int)std::min<int64_t>(steps, N);
+    std::vector<double> v(N), vprev(N, 0.0), w(N);
+    // deterministic start vector
+    double nrm = 0;
+    for (int64_t i = 0; i < N; i++) {
+        v[i] = 1.0 + 0.5 * std::sin(0.7 * (double)i);
+        nrm += v[i] * v[i];
+    }
+    nrm = std::sqrt(nrm);
+    for (auto &x : v)
+        x /= nrm;
+    std::vector<double> alpha, beta;
+    double b_prev = 0;
+    for (int k = 0; k < steps; k++) {
+        for (int64_t i = 0; i < N; i++) {
+            double s = 0;
+            for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
+                s += hval[j] * d[hcol[j]] * v[hcol[j]];
+            w[i] = d[i] * s;
+        }
+        double a = 0;
+        for (int64_t i = 0; i < N; i++)
+            a += w[i] * v[i];
+        for (int64_t i = 0; i < N; i++)
+            w[i] -= a * v[i] + b_prev * vprev[i];
+        double b = 0;
+        for (int64_t i = 0; i < N; i++)
+            b += w[i] * w[i];
+        b = std::sqrt(b);
+        alpha.push_back(a);
+        if (b < 1e-14 * std::fabs(a) || k == steps - 1)
+            break;
+        beta.push_back(b);
+        for (int64_t i = 0; i < N; i++) {
+            vprev[i] = v[i];
+            v[i] = w[i] / b;
+        }
+        b_prev = b;
+    }
+    // eigenvalues of the tridiagonal matrix by bisection (Sturm sequences)
+    const int m = (int)alpha.size();
+    auto count_below = [&](double x) {
+        int c = 0;
+        double q = alpha[0] - x;
+        if (q < 0)
+            c++;
+        for (int k = 1; k < m; k++) {
+            double bb = beta[k - 1] * beta[k - 1];
+            q = alpha[k] - x - bb / (q != 0 ? q : 1e-300);
+            if (q < 0)
+                c++;
+        }
+        return c;
+    };
+    double lo = 1e300, hi = -1e300;
+    for (int k = 0; k < m; k++) {
+        double r = (k > 0 ? std::fabs(beta[k - 1]) : 0) + (k < m - 1 ? std::fabs(beta[k]) : 0);
+        lo = std::min(lo, alpha[k] - r);
+        hi = std::max(hi, alpha[k] + r);
+    }
+    auto kth = [&](int k) {
+        double a = lo, b = hi;
+        for (int it = 0; it < 200; it++) {
+            double mid = 0.5 * (a + b);
+            if (count_below(mid) > k)
+                b = mid;
+            else
+                a = mid;
+        }
+        return 0.5 * (a + b);
+    };
+    min_eig = kth(0);
+    max_eig = kth(m - 1);
+}
+
+template <typename T>
+void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const RedArgs &r,
+          const FinArgs<T> &f) {
+    hipStream_t s = Runtime::get().stream;
+    a.row = A.row.as<uint32_t>();
+    a.col = A.col.as<uint32_t>();
+    a.n = A.n;
+    if constexpr (std::is_same<T, double>::value) {
+        XAMG_REQUIRE(vp == Prec::F64, "FP64 vectors need FP64 matrix values");
+        a.val = A.values(Prec::F64);
+        launch_spmv<double, double>(epi, nd, nv, a, r, f, s);
+    } else {
+        if (vp == Prec::BF16) {
+            a.val = A.values(Prec::BF16);
+            launch_spmv<__hip_bfloat16, float>(epi, nd, nv, a, r, f, s);
+        } else {
+            a.val = A.values(Prec::F32);
+            launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
+        }
+    }
+}
+
+template <typename T>
+void ew(int op, int nd, int nv, const EwArgs<T> &a, const RedArgs &r, const FinArgs<T> &f) {
+    launch_ew<T>(op, nd, nv, a, r, f, Runtime::get().stream);
+}
+
+template <typename T>
+void fill(T *z, int64_t n, int nv, T v, const int *halt) {
+    EwArgs<T> a;
+    a.n = n;
+    a.z = z;
+    a.c0 = v;
+    ew<T>(EW_FILL, nv, a, halt);
+}
+
+template <typename T>
+void state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap) {
+    launch_state_init<T>(ks, cs, nv, hist_cap, Runtime::get().stream);
+}
+
+template void spmv<double>(DevMatrix &, Prec, int, int, int, SpmvArgs<double>, const RedArgs &,
+                           const FinArgs<double> &);
+template void spmv<float>(DevMatrix &, Prec, int, int, int, SpmvArgs<float>, const RedArgs &,
+                          const FinArgs<float> &);
+template void ew<double>(int, int, int, const EwArgs<double> &, const RedArgs &,
+                         const FinArgs<double> &);
+template void ew<float>(int, int, int, const EwArgs<float> &, const RedArgs &,
+                        const FinArgs<float> &);
+template void fill<double>(double *, int64_t, int, double, const int *);
+template void fill<float>(float *, int64_t, int, float, const int *);
+template void state_init<double>(KrylovScalars<double> *, ConvState *, int, int);
+template void state_init<float>(KrylovScalars<float> *, ConvState *, int, int);
+
+} // namespace xamg

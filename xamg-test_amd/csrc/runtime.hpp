@@ -1,0 +1,119 @@
+// Host runtime of the MI355X XAMG path: HBM buffers, the stream, the reduction
+// workspace, and the device copy of a CSR matrix (src/mem + src/primitives/matrix
+// collapsed to one GPU holding the whole matrix).
+#pragma once
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <vector>
+
+namespace xamg {
+
+// Owning HBM allocation (mem.h:87-126 alloc_buffer, rebuilt on hipMalloc: 256-byte
+// aligned device memory instead of malloc / POSIX shm segments).
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    explicit DBuf(size_t n) { alloc(n); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0)
+            n = 256;
+        XAMG_HIP_CHECK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release() {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return (T *)p;
+    }
+};
+
+struct Runtime {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    DBuf partials, counter, result;
+    ReduceWs ws{};
+    int *pinned_flags = nullptr; // host-pinned readback slots
+    static Runtime &get();
+    void init();
+    void sync() { XAMG_HIP_CHECK(hipStreamSynchronize(stream)); }
+    bool ready = false;
+};
+
+// Device-resident CSR (csr_matrix<F,u32,u32,u32,u32>): int32 indices shared by every
+// value precision; FP32 / BF16 value copies and diagonal inverses are built on demand
+// at solver setup (csr_matrix.h:323-389, matrix/operations/convert.inl).
+struct DevMatrix {
+    int64_t n = 0, ncols = 0, nnz = 0;
+    std::vector<uint32_t> hrow, hcol;
+    std::vector<double> hval;
+    DBuf row, col, val64, val32, valbf;
+    DBuf inv_diag64, inv_sqrt_diag64, inv_diag32, inv_sqrt_diag32;
+    DBuf dense_inv64, dense_inv32; // Direct: column-major inverse (inverse.inl:108-168)
+    double min_eig = std::numeric_limits<double>::quiet_NaN();
+    double max_eig = std::numeric_limits<double>::quiet_NaN();
+
+    void upload();
+    const void *values(Prec p);
+    template <typename T>
+    const T *inv_diag();
+    template <typename T>
+    const T *inv_sqrt_diag();
+    template <typename T>
+    const T *dense_inverse();
+    // Extreme eigenvalues of D^{-1/2} A D^{-1/2} by Lanczos on the host (used when the
+    // caller supplies no bounds; hypre's setup supplies them in the reference).
+    void estimate_eigs(int steps = 60);
+};
+
+// y = A x dispatch on the value precision of the matrix copy in use.
+template <typename T>
+void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const RedArgs &r,
+          const FinArgs<T> &f);
+
+RedArgs red_args(const int *halt = nullptr);
+
+template <typename T>
+void ew(int op, int nd, int nv, const EwArgs<T> &a, const RedArgs &r, const FinArgs<T> &f);
+
+template <typename T>
+inline void ew(int op, int nv, const EwArgs<T> &a, const int *halt) {
+    ew<T>(op, 0, nv, a, red_args(halt), FinArgs<T>{});
+}
+
+template <typename T>
+void fill(T *z, int64_t n, int nv, T v, const int *halt = nullptr);
+
+// Reset a solver's device state: convergence record cleared, mask set to ones.
+template <typename T>
+void state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap);
+
+} // namespace xamg

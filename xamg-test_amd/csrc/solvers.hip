@@ -1,0 +1,892 @@
+// Solver implementations (see solvers.hpp).  Each apply() only enqueues work on the
+// runtime stream; scalars, masks and convergence state live in HBM.
+#include "solvers.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <set>
+#include <sstream>
+
+namespace xamg {
+
+EngineCfg &engine_cfg() {
+    static EngineCfg c;
+    return c;
+}
+
+static std::set<std::string> split_set(const std::string &s) {
+    std::set<std::string> out;
+    std::stringstream ss(s);
+    std::string t;
+    while (std::getline(ss, t, ','))
+        if (!t.empty())
+            out.insert(t);
+    return out;
+}
+
+int run_loop(int max_iters, ConvState *cs, const std::function<void()> &body) {
+    Runtime &rt = Runtime::get();
+    static hipEvent_t ev[2] = {nullptr, nullptr};
+    if (!ev[0]) {
+        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
+    int K = std::max(1, engine_cfg().batch);
+    int it = 0, batches = 0, pending = -1;
+    while (it < max_iters) {
+        int k = std::min(K, max_iters - it);
+        for (int i = 0; i < k; i++)
+            body();
+        it += k;
+        batches++;
+        if (cs == nullptr)
+            continue;
+        int slot = batches & 1;
+        XAMG_HIP_CHECK(hipMemcpyAsync(&rt.pinned_flags[slot], &cs->halt, sizeof(int),
+                                      hipMemcpyDeviceToHost, rt.stream));
+        XAMG_HIP_CHECK(hipEventRecord(ev[slot], rt.stream));
+        if (pending >= 0) {
+            XAMG_HIP_CHECK(hipEventSynchronize(ev[pending]));
+            if (rt.pinned_flags[pending])
+                break;
+        }
+        pending = slot;
+    }
+    if (cs != nullptr && pending >= 0)
+        XAMG_HIP_CHECK(hipEventSynchronize(ev[pending]));
+    return batches;
+}
+
+// ---------------------------------------------------------------------------
+// Solver base
+// ---------------------------------------------------------------------------
+
+template <typename T>
+void Solver<T>::read_conf() {
+    conv_check = 0;
+    if (prm.has("convergence_details")) {
+        auto cd = split_set(prm.get_string("convergence_details"));
+        if (!cd.count("nocheck") && cd.count("check-L2"))
+            conv_check = 1;
+    }
+    // stats.h:117-122: tolerances apply only when both are set
+    if (prm.has("abs_tolerance") && prm.has("rel_tolerance")) {
+        abs_tol = prm.get_float("abs_tolerance");
+        rel_tol = prm.get_float("rel_tolerance");
+    } else {
+        abs_tol = rel_tol = 0.0;
+    }
+    max_iters = prm.has("max_iters") ? (int)prm.get_int("max_iters") : 1;
+}
+
+template <typename T>
+void Solver<T>::setup() {
+    read_conf();
+    state.alloc(sizeof(ConvState) + sizeof(KrylovScalars<T>) + 256);
+    cs = state.as<ConvState>();
+    ks = (KrylovScalars<T> *)((char *)state.p + ((sizeof(ConvState) + 255) / 256) * 256);
+    hist_cap = conv_check ? std::max(2, max_iters + 2) : 0;
+    if (hist_cap > 0)
+        histbuf.alloc(sizeof(double) * hist_cap * nv);
+    state_init<T>(ks, cs, nv, hist_cap);
+}
+
+template <typename T>
+void Solver<T>::alloc_bufs(int k) {
+    bufs.clear();
+    for (int i = 0; i < k; i++)
+        bufs.emplace_back(sizeof(T) * std::max<int64_t>(A->n, 1) * nv);
+    for (int i = 0; i < k; i++)
+        fill<T>(vec(i), A->n, nv, (T)0);
+}
+
+template <typename T>
+FinArgs<T> Solver<T>::fin(int f, bool check_mask) {
+    FinArgs<T> a;
+    a.ks = ks;
+    a.cs = cs;
+    a.hist = hist_cap > 0 ? histbuf.as<double>() : nullptr;
+    a.fin = f;
+    a.conv_check = conv_check;
+    a.abs_tol = abs_tol;
+    a.rel_tol = rel_tol;
+    a.iconv = check_mask ? ks->iconv : nullptr;
+    return a;
+}
+
+template <typename T>
+void Solver<T>::stats(SolveStats &st, bool with_hist) {
+    Runtime &rt = Runtime::get();
+    rt.sync();
+    ConvState h;
+    XAMG_HIP_CHECK(hipMemcpy(&h, cs, sizeof(ConvState), hipMemcpyDeviceToHost));
+    st.iters = h.iter;
+    st.sub_iters = h.sub_iter;
+    for (int l = 0; l < nv; l++) {
+        st.converged[l] = h.status[l];
+        st.res0[l] = h.res0[l];
+        st.res[l] = h.res[l];
+    }
+    st.nhist = h.nhist;
+    if (with_hist && h.nhist > 0) {
+        st.hist.resize((size_t)h.nhist * nv);
+        XAMG_HIP_CHECK(hipMemcpy(st.hist.data(), histbuf.p, sizeof(double) * h.nhist * nv,
+                                 hipMemcpyDeviceToHost));
+    }
+}
+
+template <typename T>
+void Solver<T>::renew(const GlobalParams &g) {
+    if (g.has(role))
+        for (auto &kv : g.get(role).kv)
+            prm.set(kv.first, kv.second);
+    int old_cap = hist_cap;
+    read_conf();
+    int need = conv_check ? std::max(2, max_iters + 2) : 0;
+    if (need > old_cap) {
+        hist_cap = need;
+        histbuf.alloc(sizeof(double) * hist_cap * nv);
+    }
+}
+
+static Prec value_prec_for(const ParamList &p, bool single) {
+    if (!single)
+        return Prec::F64;
+    if (p.has("values_precision")) {
+        std::string v = p.get_string("values_precision");
+        if (v == "bf16")
+            return Prec::BF16;
+    }
+    return Prec::F32;
+}
+
+// ---------------------------------------------------------------------------
+// Identity (solver.h:299-323) and Direct (direct.inl:31-59)
+// ---------------------------------------------------------------------------
+template <typename T>
+class Identity : public Solver<T> {
+  public:
+    void apply(T *x, bool, const T *b, const int *halt) override {
+        EwArgs<T> a;
+        a.n = this->A->n;
+        a.z = x;
+        a.x = b;
+        a.y2 = x;
+        ew<T>(EW_COPY3, this->nv, a, halt);
+    }
+};
+
+template <typename T>
+class Direct : public Solver<T> {
+  public:
+    void setup() override {
+        Solver<T>::setup();
+        this->A->template dense_inverse<T>();
+    }
+    void apply(T *x, bool, const T *b, const int *halt) override {
+        launch_dense_mv<T>(this->A->template dense_inverse<T>(), this->A->n, b, x, this->nv, halt,
+                           Runtime::get().stream);
+    }
+};
+
+// Per-RHS coefficient array helper (relax factors): lives next to the solver state.
+template <typename T>
+static DBuf const_coef(int nv, T v) {
+    std::vector<T> h(kMaxNV, v);
+    DBuf b(sizeof(T) * kMaxNV);
+    XAMG_HIP_CHECK(hipMemcpy(b.p, h.data(), sizeof(T) * kMaxNV, hipMemcpyHostToDevice));
+    (void)nv;
+    return b;
+}
+
+// Residual r = b - A x with ||r||^2 reduced and checked (stats.h:195-233 / is_converged).
+template <typename T>
+static void resid_norm(Solver<T> *S, T *x, bool xzero, const T *b, T *r, int finop,
+                       const int *halt, bool mask) {
+    if (xzero) {
+        EwArgs<T> a;
+        a.n = S->A->n;
+        a.z = r;
+        a.x = b;
+        a.y2 = r;
+        RedArgs ra = red_args(halt);
+        ew<T>(EW_COPY3, 1, S->nv, a, ra, S->fin(finop, mask));
+        return;
+    }
+    SpmvArgs<T> a{};
+    a.x = x;
+    a.y = r;
+    a.b = b;
+    spmv<T>(*S->A, S->vprec, EPI_RESID, 1, S->nv, a, red_args(halt), S->fin(finop, mask));
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi (jacobi.inl:51-102)
+// ---------------------------------------------------------------------------
+template <typename T>
+class Jacobi : public Solver<T> {
+    DBuf relax;
+
+  public:
+    void setup() override {
+        Solver<T>::setup();
+        this->alloc_bufs(1);
+        relax = const_coef<T>(this->nv, (T)this->prm.get_float("relax_factor"));
+        this->A->template inv_diag<T>();
+    }
+    void apply(T *x, bool xzero, const T *b, const int *halt) override {
+        auto *A = this->A;
+        const int nv = this->nv;
+        const T *d = A->template inv_diag<T>();
+        T *tmp = this->vec(0);
+        if (this->conv_check) {
+            state_init<T>(this->ks, this->cs, nv, this->hist_cap);
+            const int *H = &this->cs->halt;
+            T *r = tmp;
+            resid_norm<T>(this, x, xzero, b, r, FIN_NORM_INIT, nullptr, true);
+            if (xzero)
+                fill<T>(x, A->n, nv, (T)0);
+            run_loop(this->max_iters, this->cs, [&]() {
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = x;
+                a.y = r;
+                a.d = d;
+                a.a = relax.as<T>();
+                a.c = this->ks->iconv;
+                ew<T>(EW_JAC_UPD, nv, a, H);
+                resid_norm<T>(this, x, false, b, r, FIN_NORM, H, true);
+            });
+            return;
+        }
+        // no-check mode: k sweeps r = A x; x += w D^{-1} (b - r), out of place so the
+        // gathered neighbours always hold the previous sweep.
+        const int k = this->max_iters;
+        if (k <= 0)
+            return;
+        const T *src = x;
+        int s0 = 1;
+        if (xzero) {
+            T *dst = ((k - 1) % 2 == 0) ? x : tmp;
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = dst;
+            a.x = b;
+            a.d = d;
+            a.a = relax.as<T>();
+            ew<T>(EW_JACOBI0, nv, a, halt);
+            src = dst;
+            s0 = 2;
+        } else if (k % 2 == 1) {
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = tmp;
+            a.x = x;
+            a.y2 = tmp;
+            ew<T>(EW_COPY3, nv, a, halt);
+            src = tmp;
+        }
+        for (int s = s0; s <= k; s++) {
+            T *dst = ((k - s) % 2 == 0) ? x : tmp;
+            SpmvArgs<T> a{};
+            a.x = src;
+            a.y = dst;
+            a.xin = src;
+            a.b = b;
+            a.d = d;
+            a.wcoef = relax.as<T>();
+            spmv<T>(*A, this->vprec, EPI_JACOBI, 0, nv, a, red_args(halt), FinArgs<T>{});
+            src = dst;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Chebyshev (chebyshev.inl:45-183): Horner form in D^{-1/2} A D^{-1/2}
+// ---------------------------------------------------------------------------
+template <typename T>
+class Chebyshev : public Solver<T> {
+    T c[4];
+    int order = 2;
+
+  public:
+    void setup() override {
+        Solver<T>::setup();
+        this->alloc_bufs(3);
+        auto *A = this->A;
+        A->template inv_sqrt_diag<T>();
+        order = (int)this->prm.get_int("polynomial_order");
+        XAMG_REQUIRE(order >= 1 && order <= 4, "Chebyshev polynomial_order must be 1..4");
+        double lo = A->min_eig, hi = A->max_eig;
+        if (this->prm.has("min_eig"))
+            lo = this->prm.get_float("min_eig");
+        if (this->prm.has("max_eig"))
+            hi = this->prm.get_float("max_eig");
+        if (std::isnan(lo) || std::isnan(hi)) {
+            A->estimate_eigs();
+            lo = A->min_eig;
+            hi = A->max_eig;
+        }
+        double frac = this->prm.get_float("spectrum_fraction");
+        // chebyshev.inl:67-125
+        double den, cc[4] = {0, 0, 0, 0};
+        double upper = hi * 1.1;
+        double lower = (upper - lo) * frac + lo;
+        double theta = (upper + lower) / 2.;
+        double delta = (upper - lower) / 2.;
+        switch (order) {
+        case 1:
+            cc[0] = 1. / theta;
+            break;
+        case 2:
+            den = delta * delta - 2 * theta * theta;
+            cc[0] = -4 * theta / den;
+            cc[1] = 2 / den;
+            break;
+        case 3:
+            den = 3 * (delta * delta) * theta - 4 * (theta * theta * theta);
+            cc[0] = (3 * delta * delta - 12 * theta * theta) / den;
+            cc[1] = 12 * theta / den;
+            cc[2] = -4 / den;
+            break;
+        default:
+            den = std::pow(delta, 4) - 8 * delta * delta * theta * theta + 8 * std::pow(theta, 4);
+            cc[0] = (32 * std::pow(theta, 3) - 16 * delta * delta * theta) / den;
+            cc[1] = (8 * delta * delta - 48 * theta * theta) / den;
+            cc[2] = 32 * theta / den;
+            cc[3] = -8 / den;
+            break;
+        }
+        for (int q = 0; q < 4; q++)
+            c[q] = (T)cc[q];
+    }
+    void apply(T *x, bool xzero, const T *b, const int *halt) override {
+        auto *A = this->A;
+        const int nv = this->nv;
+        const T *d = A->template inv_sqrt_diag<T>();
+        const int *H = halt;
+        if (this->conv_check) {
+            state_init<T>(this->ks, this->cs, nv, this->hist_cap);
+            H = &this->cs->halt;
+        }
+        const T *in = b;
+        if (!xzero || this->conv_check) {
+            T *R = this->vec(0);
+            if (this->conv_check)
+                resid_norm<T>(this, x, xzero, b, R, FIN_NORM_INIT, halt, false);
+            else {
+                SpmvArgs<T> a{};
+                a.x = x;
+                a.y = R;
+                a.b = b;
+                spmv<T>(*A, this->vprec, EPI_RESID, 0, nv, a, red_args(H), FinArgs<T>{});
+            }
+            in = R;
+        }
+        if (order == 1) {
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = x;
+            a.x = in;
+            a.d = d;
+            a.c0 = c[0];
+            a.y = xzero ? nullptr : x;
+            ew<T>(EW_CHEB0_LAST, nv, a, H);
+        } else {
+            T *t1 = this->vec(1), *t2 = this->vec(2);
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = t1;
+            a.x = in;
+            a.d = d;
+            a.c0 = c[order - 1];
+            ew<T>(EW_CHEB0, nv, a, H);
+            for (int q = order - 2; q >= 0; --q) {
+                SpmvArgs<T> s{};
+                s.x = t1;
+                s.b = in;
+                s.d = d;
+                s.dcol = d;
+                s.coef = c[q];
+                if (q > 0) {
+                    s.y = t2;
+                    spmv<T>(*A, this->vprec, EPI_CHEB, 0, nv, s, red_args(H), FinArgs<T>{});
+                    std::swap(t1, t2);
+                } else {
+                    s.y = x;
+                    s.xin = xzero ? nullptr : x;
+                    spmv<T>(*A, this->vprec, EPI_CHEB_LAST, 0, nv, s, red_args(H), FinArgs<T>{});
+                }
+            }
+        }
+        if (this->conv_check) {
+            // stats.init(r) without a mask: Chebyshev never masks RHS updates
+            resid_norm<T>(this, x, false, b, this->vec(0), FIN_NORM, H, false);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// HSGS (hsgs.inl:52-91) with the block-hybrid sweep of blas2::SGS
+// ---------------------------------------------------------------------------
+template <typename T>
+class HSGS : public Solver<T> {
+    DBuf relax;
+    int nblocks = 1;
+
+  public:
+    void setup() override {
+        Solver<T>::setup();
+        this->alloc_bufs(2);
+        relax = const_coef<T>(this->nv, (T)this->prm.get_float("relax_factor"));
+        this->A->template inv_diag<T>();
+        if (this->prm.has("gs_blocks"))
+            nblocks = (int)this->prm.get_int("gs_blocks");
+        else
+            nblocks = (int)std::max<int64_t>(1, this->A->n / 128);
+    }
+    void sweep(T *x, const T *b, const int *H, const T *mask) {
+        auto *A = this->A;
+        XAMG_REQUIRE(this->vprec != Prec::BF16, "HSGS supports FP64/FP32 values");
+        const void *val = A->values(this->vprec);
+        hipStream_t s = Runtime::get().stream;
+        if constexpr (std::is_same<T, double>::value)
+            launch_hsgs<double, double>(A->row.template as<uint32_t>(), A->col.template as<uint32_t>(),
+                                        (const double *)val, A->template inv_diag<T>(), A->n,
+                                        this->nv, b, x, this->vec(1), relax.as<T>(), mask, nblocks,
+                                        H, s);
+        else
+            launch_hsgs<float, float>(A->row.template as<uint32_t>(), A->col.template as<uint32_t>(),
+                                      (const float *)val, A->template inv_diag<T>(), A->n, this->nv,
+                                      b, x, this->vec(1), relax.as<T>(), mask, nblocks, H, s);
+    }
+    void apply(T *x, bool xzero, const T *b, const int *halt) override {
+        const int nv = this->nv;
+        if (xzero)
+            fill<T>(x, this->A->n, nv, (T)0, halt);
+        if (this->conv_check) {
+            state_init<T>(this->ks, this->cs, nv, this->hist_cap);
+            const int *H = &this->cs->halt;
+            T *r = this->vec(0);
+            resid_norm<T>(this, x, false, b, r, FIN_NORM_INIT, nullptr, true);
+            run_loop(this->max_iters, this->cs, [&]() {
+                sweep(x, b, H, this->ks->iconv);
+                resid_norm<T>(this, x, false, b, r, FIN_NORM, H, true);
+            });
+            return;
+        }
+        for (int it = 0; it < this->max_iters; it++)
+            sweep(x, b, halt, nullptr);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// BiCGStab / PBiCGStab (bicgstab.inl:43-197, pbicgstab.inl:47-207)
+// ---------------------------------------------------------------------------
+template <typename T>
+class Krylov : public Solver<T> {
+  public:
+    bool pre = false;
+    std::unique_ptr<Solver<T>> precond;
+    std::shared_ptr<MgHierarchy> hier;
+
+    void setup() override {
+        Solver<T>::setup();
+        this->alloc_bufs(pre ? 8 : 6);
+        if (pre) {
+            XAMG_REQUIRE(this->gp->has("preconditioner"), "PBiCGStab needs a preconditioner block");
+            const ParamList &pp = this->gp->get("preconditioner");
+            precond = make_solver<T>(pp.get_string("method"), this->A, pp, "preconditioner",
+                                     this->nv, this->gp, this->level, hier);
+        }
+    }
+    void renew(const GlobalParams &g) override {
+        Solver<T>::renew(g);
+        if (precond)
+            precond->renew(g);
+    }
+    void apply(T *x, bool xzero, const T *b, const int *halt) override {
+        auto *A = this->A;
+        const int nv = this->nv;
+        auto *ks = this->ks;
+        auto *cs = this->cs;
+        state_init<T>(ks, cs, nv, this->hist_cap);
+        T *r0 = this->vec(0), *r = this->vec(1), *p = this->vec(2), *v = this->vec(3),
+          *s = this->vec(4), *t = this->vec(5);
+        T *p_ = pre ? this->vec(6) : p, *s_ = pre ? this->vec(7) : s;
+        const int *H = this->conv_check ? &cs->halt : halt;
+        // 1-3: r0 = b - A x0, rho = (r0, r0), r = p = r0
+        if (xzero) {
+            fill<T>(x, A->n, nv, (T)0, halt);
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = r0;
+            a.x = b;
+            a.y2 = r;
+            a.y3 = p;
+            ew<T>(EW_COPY3, 1, nv, a, red_args(halt), this->fin(FIN_INIT, true));
+        } else {
+            SpmvArgs<T> a{};
+            a.x = x;
+            a.y = r0;
+            a.b = b;
+            a.y2 = r;
+            a.y3 = p;
+            spmv<T>(*A, this->vprec, EPI_RESID, 1, nv, a, red_args(halt), this->fin(FIN_INIT, true));
+        }
+        auto iteration = [&]() {
+            if (pre)
+                precond->apply(p_, true, p, H);
+            { // v = A p_, alpha = rho / (v, r0)
+                SpmvArgs<T> a{};
+                a.x = p_;
+                a.y = v;
+                a.w = r0;
+                spmv<T>(*A, this->vprec, EPI_AXY, 1, nv, a, red_args(H), this->fin(FIN_ALPHA));
+            }
+            { // s = r - alpha v
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = s;
+                a.x = r;
+                a.y = v;
+                a.b = ks->nalpha;
+                ew<T>(EW_AXPBY_Z, nv, a, H);
+            }
+            if (pre)
+                precond->apply(s_, true, s, H);
+            { // t = A s_, (t,s), (t,t), (s,s), omega, convergence check
+                SpmvArgs<T> a{};
+                a.x = s_;
+                a.y = t;
+                a.w = s;
+                spmv<T>(*A, this->vprec, EPI_AXY, 3, nv, a, red_args(H),
+                        this->fin(FIN_OMEGA, true));
+            }
+            { // x = x + alpha_c p_ + omega_c s_
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = x;
+                a.x = p_;
+                a.a = ks->ac;
+                a.y = s_;
+                a.b = ks->oc;
+                ew<T>(EW_AXPBYPCZ, nv, a, H);
+            }
+            { // r = s - omega t, rho_n = (r, r0), beta
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = r;
+                a.x = s;
+                a.y = t;
+                a.b = ks->nomega;
+                a.w = r0;
+                RedArgs ra = red_args(H);
+                ra.set_halt_if_done = this->conv_check;
+                ra.cs = cs;
+                ew<T>(EW_AXPBY_Z, 1, nv, a, ra, this->fin(FIN_BETA));
+            }
+            { // p = r + beta p - beta omega v
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = p;
+                a.x = r;
+                a.y = v;
+                a.b = ks->nbo;
+                a.c = ks->beta;
+                ew<T>(EW_AXPBYPCZ, nv, a, H);
+            }
+        };
+        if (this->conv_check)
+            run_loop(this->max_iters, cs, iteration);
+        else
+            for (int i = 0; i < this->max_iters; i++)
+                iteration();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// IterativeRefinement (ir.inl:61-194), FP64 outer loop
+// ---------------------------------------------------------------------------
+class IterRefine : public Solver<double> {
+  public:
+    bool reduced = false;
+    std::unique_ptr<Solver<float>> inner32;
+    std::unique_ptr<Solver<double>> inner64;
+    DBuf ix32, ib32;
+    int sub_iters = 0;
+
+    void setup() override {
+        Solver<double>::setup();
+        if (!prm.has("max_iters"))
+            max_iters = 20;
+        alloc_bufs(2);
+        reduced = prm.has("subsolver_reduced_precision") && prm.get_bool("subsolver_reduced_precision");
+        XAMG_REQUIRE(gp->has("solver"), "IterativeRefinement needs a solver block");
+        const ParamList &sp = gp->get("solver");
+        if (reduced) {
+            inner32 = make_solver<float>(sp.get_string("method"), A, sp, "solver", nv, gp, 0);
+            ix32.alloc(sizeof(float) * std::max<int64_t>(A->n, 1) * nv);
+            ib32.alloc(sizeof(float) * std::max<int64_t>(A->n, 1) * nv);
+        } else {
+            inner64 = make_solver<double>(sp.get_string("method"), A, sp, "solver", nv, gp, 0);
+        }
+    }
+    void renew(const GlobalParams &g) override {
+        Solver<double>::renew(g);
+        if (!prm.has("max_iters"))
+            max_iters = 20;
+        if (inner32)
+            inner32->renew(g);
+        if (inner64)
+            inner64->renew(g);
+    }
+    int inner_iters() {
+        ConvState h;
+        ConvState *c = inner32 ? inner32->cs : inner64->cs;
+        XAMG_HIP_CHECK(hipMemcpyAsync(&h, c, sizeof(ConvState), hipMemcpyDeviceToHost,
+                                      Runtime::get().stream));
+        Runtime::get().sync();
+        return h.iter;
+    }
+    void apply(double *x, bool xzero, const double *b, const int *) override {
+        Runtime &rt = Runtime::get();
+        state_init<double>(ks, cs, nv, hist_cap);
+        double *r = vec(0), *xc = vec(1);
+        if (xzero)
+            fill<double>(x, A->n, nv, 0.0);
+        resid_norm<double>(this, x, false, b, r, FIN_NORM_INIT, nullptr, false);
+        sub_iters = 0;
+        ConvState h;
+        XAMG_HIP_CHECK(hipMemcpyAsync(&h, cs, sizeof(ConvState), hipMemcpyDeviceToHost, rt.stream));
+        rt.sync();
+        if (h.halt)
+            return;
+        for (int it = 1; it <= max_iters; it++) {
+            if (reduced) {
+                EwArgs<float> c;
+                c.n = A->n;
+                c.z = ib32.as<float>();
+                c.xo = r;
+                ew<float>(EW_CONVERT, nv, c, nullptr);
+                inner32->apply(ix32.as<float>(), true, ib32.as<float>(), nullptr);
+                EwArgs<double> a;
+                a.n = A->n;
+                a.z = x;
+                a.xo = ix32.p;
+                ew<double>(EW_ADD_CONV, nv, a, nullptr);
+            } else {
+                inner64->apply(xc, true, r, nullptr);
+                EwArgs<double> a;
+                a.n = A->n;
+                a.z = x;
+                a.x = xc;
+                a.y = x;
+                ew<double>(EW_AXPBY_Z, nv, a, nullptr);
+            }
+            sub_iters += inner_iters();
+            resid_norm<double>(this, x, false, b, r, FIN_NORM, nullptr, false);
+            XAMG_HIP_CHECK(hipMemcpyAsync(&h, cs, sizeof(ConvState), hipMemcpyDeviceToHost, rt.stream));
+            rt.sync();
+            if (h.halt)
+                break;
+        }
+    }
+    void stats(SolveStats &st, bool with_hist) override {
+        Solver<double>::stats(st, with_hist);
+        st.sub_iters = sub_iters;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// MultiGrid (mg.inl:117-331) over a host-built hierarchy
+// ---------------------------------------------------------------------------
+template <typename T>
+class MultiGrid : public Solver<T> {
+  public:
+    std::shared_ptr<MgHierarchy> hier;
+    std::vector<std::unique_ptr<Solver<T>>> pre, post;
+    std::unique_ptr<Solver<T>> coarse;
+    std::vector<DBuf> temp, bH, xH;
+    int cycle = 0;
+
+    void setup() override {
+        Solver<T>::setup();
+        this->alloc_bufs(1);
+        if (!hier)
+            hier = build_amg(this->A, this->prm);
+        auto &L = hier->levels;
+        const int nl = (int)L.size();
+        XAMG_REQUIRE(nl >= 2, "MultiGrid hierarchy needs at least two levels");
+        cycle = (int)this->prm.get_int("mg_cycle");
+        const GlobalParams *g = this->gp;
+        for (int l = 0; l < nl - 1; l++) {
+            ParamList pp = g->get_level("pre_smoother", l), qp = g->get_level("post_smoother", l);
+            pre.push_back(make_solver<T>(pp.get_string("method"), L[l].A, pp, "pre_smoother",
+                                         this->nv, g, l));
+            post.push_back(make_solver<T>(qp.get_string("method"), L[l].A, qp, "post_smoother",
+                                          this->nv, g, l));
+        }
+        const ParamList &cp = g->get("coarse_grid_solver");
+        coarse = make_solver<T>(cp.get_string("method"), L[nl - 1].A, cp, "coarse_grid_solver",
+                                this->nv, g, nl - 1);
+        temp.resize(nl);
+        bH.resize(nl);
+        xH.resize(nl);
+        for (int l = 0; l < nl; l++) {
+            size_t sz = sizeof(T) * std::max<int64_t>(L[l].A->n, 1) * this->nv;
+            temp[l].alloc(sz);
+            bH[l].alloc(sz);
+            xH[l].alloc(sz);
+        }
+    }
+    void renew(const GlobalParams &g) override {
+        Solver<T>::renew(g);
+        for (auto &s : pre)
+            s->renew(g);
+        for (auto &s : post)
+            s->renew(g);
+        coarse->renew(g);
+    }
+    // mg.inl:186-280 MG_cycle (uniform precision)
+    void mg_cycle(T *x, bool xzero, const T *b, int lev, int cyc, const int *H) {
+        auto &L = hier->levels;
+        const int nv = this->nv;
+        DevMatrix *A = L[lev].A;
+        T *tmp = temp[lev].template as<T>();
+        T *bh = bH[lev + 1].template as<T>();
+        T *xh = xH[lev + 1].template as<T>();
+        if (cyc != 1) {
+            pre[lev]->apply(x, xzero, b, H);
+            xzero = false;
+        }
+        if (xzero) {
+            // temp = 1*b (A*0 = 0)
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = tmp;
+            a.x = b;
+            a.y2 = tmp;
+            ew<T>(EW_COPY3, nv, a, H);
+        } else {
+            SpmvArgs<T> a{};
+            a.x = x;
+            a.y = tmp;
+            a.b = b;
+            spmv<T>(*A, this->vprec, EPI_RESID, 0, nv, a, red_args(H), FinArgs<T>{});
+        }
+        {
+            SpmvArgs<T> a{};
+            a.x = tmp;
+            a.y = bh;
+            spmv<T>(*L[lev].R, this->vprec, EPI_AXY, 0, nv, a, red_args(H), FinArgs<T>{});
+        }
+        if (lev + 1 == (int)L.size() - 1) {
+            coarse->apply(xh, true, bh, H);
+        } else {
+            mg_cycle(xh, true, bh, lev + 1, cyc, H);
+            if (cyc == 2)
+                mg_cycle(xh, false, bh, lev + 1, 2, H);
+            else if (cyc == 1)
+                mg_cycle(xh, false, bh, lev + 1, 0, H);
+        }
+        if (xzero)
+            fill<T>(x, A->n, nv, (T)0, H);
+        {
+            SpmvArgs<T> a{};
+            a.x = xh;
+            a.y = x;
+            spmv<T>(*L[lev].P, this->vprec, EPI_AXPY, 0, nv, a, red_args(H), FinArgs<T>{});
+        }
+        post[lev]->apply(x, false, b, H);
+    }
+    void apply(T *x, bool xzero, const T *b, const int *halt) override {
+        const int nv = this->nv;
+        if (this->conv_check) {
+            state_init<T>(this->ks, this->cs, nv, this->hist_cap);
+            const int *H = &this->cs->halt;
+            T *r = this->vec(0);
+            resid_norm<T>(this, x, xzero, b, r, FIN_NORM_INIT, nullptr, true);
+            bool xz = xzero;
+            run_loop(this->max_iters, this->cs, [&]() {
+                mg_cycle(x, xz, b, 0, cycle, H);
+                xz = false;
+                resid_norm<T>(this, x, false, b, r, FIN_NORM, H, true);
+            });
+            return;
+        }
+        bool xz = xzero;
+        for (int it = 0; it < this->max_iters; it++) {
+            mg_cycle(x, xz, b, 0, cycle, halt);
+            xz = false;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// factory (solver.h:352-399)
+// ---------------------------------------------------------------------------
+template <typename T>
+std::unique_ptr<Solver<T>> make_solver(const std::string &method, DevMatrix *A,
+                                       const ParamList &prm, const std::string &role, int nv,
+                                       const GlobalParams *gp, int level,
+                                       std::shared_ptr<MgHierarchy> hier) {
+    std::unique_ptr<Solver<T>> s;
+    if (method == "Identity")
+        s.reset(new Identity<T>());
+    else if (method == "Direct")
+        s.reset(new Direct<T>());
+    else if (method == "Jacobi")
+        s.reset(new Jacobi<T>());
+    else if (method == "Chebyshev")
+        s.reset(new Chebyshev<T>());
+    else if (method == "HSGS")
+        s.reset(new HSGS<T>());
+    else if (method == "BiCGStab" || method == "MergedBiCGStab" || method == "PBiCGStab" ||
+             method == "MergedPBiCGStab") {
+        auto *k = new Krylov<T>();
+        k->pre = (method == "PBiCGStab" || method == "MergedPBiCGStab");
+        k->hier = hier;
+        s.reset(k);
+    } else if (method == "MultiGrid") {
+        auto *m = new MultiGrid<T>();
+        m->hier = hier;
+        s.reset(m);
+    } else if (method == "IterativeRefinement") {
+        if constexpr (std::is_same<T, double>::value)
+            s.reset(new IterRefine());
+        else
+            throw std::runtime_error("xamg: IterativeRefinement must be the FP64 meta solver");
+    } else {
+        throw std::runtime_error("xamg: the selected method '" + method + "' is not implemented");
+    }
+    s->A = A;
+    s->prm = prm;
+    s->role = role;
+    s->nv = nv;
+    s->gp = gp;
+    s->level = level;
+    s->vprec = value_prec_for(prm, std::is_same<T, float>::value);
+    s->setup();
+    return s;
+}
+
+std::unique_ptr<SolverBase> construct_solver_hierarchy(const GlobalParams &gp, DevMatrix *A, int nv,
+                                                       Solver<double> **top64) {
+    std::string role = gp.has("meta_solver") ? "meta_solver" : "solver";
+    const ParamList &p = gp.get(role);
+    auto s = make_solver<double>(p.get_string("method"), A, p, role, nv, &gp, 0);
+    *top64 = s.get();
+    return std::unique_ptr<SolverBase>(s.release());
+}
+
+template std::unique_ptr<Solver<double>> make_solver<double>(const std::string &, DevMatrix *,
+                                                             const ParamList &, const std::string &,
+                                                             int, const GlobalParams *, int,
+                                                             std::shared_ptr<MgHierarchy>);
+template std::unique_ptr<Solver<float>> make_solver<float>(const std::string &, DevMatrix *,
+                                                           const ParamList &, const std::string &,
+                                                           int, const GlobalParams *, int,
+                                                           std::shared_ptr<MgHierarchy>);
+
+} // namespace xamg

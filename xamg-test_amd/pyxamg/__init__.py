@@ -1,0 +1,398 @@
+"""ctypes binding of libxamg_c_api.so (the XAMG C API on one MI355X).
+
+The functions mirror api/c/xamg_c_api.h of the reference one to one (same names with
+the XAMG_ prefix dropped), plus the xamgx_* extensions of include/xamg_ext.h.  The
+library is the product; this module only loads it.  There is no CPU fallback: every
+call fails loudly if the shared library or the GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libxamg_c_api.so")
+_LIB = None
+
+MEM_LOCAL, MEM_SHARED, MEM_DISTRIBUTED = 0, 1, 2
+
+
+class XamgError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise XamgError(f"{LIB_PATH} is missing: build it with `make -C xamg-test_amd`")
+        L = C.CDLL(LIB_PATH)
+        sz, vp, ci, cd = C.c_size_t, C.c_void_p, C.c_int, C.c_double
+        psz = C.POINTER(C.c_size_t)
+        cp = C.c_char_p
+        sigs = {
+            "XAMG_init": (ci, [ci, vp, cp]),
+            "XAMG_finalize": (ci, []),
+            "XAMG_get_nv": (ci, []),
+            "XAMG_id_get_proc": (ci, []),
+            "XAMG_id_get_nprocs": (ci, []),
+            "XAMG_id_get_comm_ptr": (vp, []),
+            "XAMG_param_create": (ci, [psz]),
+            "XAMG_param_add_value_s": (ci, [cp, cp, cp, sz]),
+            "XAMG_param_add_value_i": (ci, [cp, cp, C.c_uint32, sz]),
+            "XAMG_param_add_value_d": (ci, [cp, cp, cd, sz]),
+            "XAMG_param_add_value_b": (ci, [cp, cp, C.c_uint8, sz]),
+            "XAMG_param_change_value_s": (ci, [cp, cp, cp, sz]),
+            "XAMG_param_change_value_i": (ci, [cp, cp, C.c_uint32, sz]),
+            "XAMG_param_change_value_d": (ci, [cp, cp, cd, sz]),
+            "XAMG_param_change_value_b": (ci, [cp, cp, C.c_uint8, sz]),
+            "XAMG_param_forced_change_value_s": (ci, [cp, cp, cp, sz]),
+            "XAMG_param_forced_change_value_i": (ci, [cp, cp, C.c_uint32, sz]),
+            "XAMG_param_forced_change_value_d": (ci, [cp, cp, cd, sz]),
+            "XAMG_param_forced_change_value_b": (ci, [cp, cp, C.c_uint8, sz]),
+            "XAMG_param_override_create": (ci, [psz]),
+            "XAMG_param_override_add_value": (ci, [cp, cp, sz]),
+            "XAMG_param_override_apply": (ci, [cp, sz, sz]),
+            "XAMG_param_override_destroy": (ci, [sz]),
+            "XAMG_param_set_default": (None, [sz]),
+            "XAMG_param_print": (ci, [sz]),
+            "XAMG_param_destroy": (ci, [sz]),
+            "XAMG_part_create": (ci, [psz]),
+            "XAMG_part_construct": (ci, [ci, sz]),
+            "XAMG_part_get_numa_size": (ci, [sz]),
+            "XAMG_part_destroy": (ci, [sz]),
+            "XAMG_vector_create": (ci, [psz, ci]),
+            "XAMG_vector_offset": (ci, [C.c_uint64, sz]),
+            "XAMG_vector_alloc_d": (ci, [ci, C.c_uint16, sz]),
+            "XAMG_vector_set_part": (ci, [sz, sz]),
+            "XAMG_vector_set_val_d": (ci, [cd, sz]),
+            "XAMG_vector_download_d": (ci, [vp, C.c_uint64, C.c_uint64, C.c_uint16, sz]),
+            "XAMG_vector_upload_v": (ci, [sz, sz]),
+            "XAMG_vector_upload_d": (ci, [vp, C.c_uint64, C.c_uint64, C.c_uint16, sz]),
+            "XAMG_vector_destroy": (ci, [sz]),
+            "XAMG_csr_matrix_create": (ci, [psz]),
+            "XAMG_csr_matrix_alloc": (ci, [C.c_uint64, C.c_uint64, C.c_uint64, sz]),
+            "XAMG_csr_matrix_fill": (ci, [vp, vp, vp, sz]),
+            "XAMG_csr_matrix_offset": (ci, [ci, ci, sz]),
+            "XAMG_csr_matrix_get_nrows": (C.c_uint32, [sz]),
+            "XAMG_csr_matrix_destroy": (ci, [sz]),
+            "XAMG_matrix_create": (ci, [psz]),
+            "XAMG_matrix_set_part": (ci, [sz, sz]),
+            "XAMG_matrix_construct": (ci, [sz, sz]),
+            "XAMG_matrix_destroy": (ci, [sz]),
+            "XAMG_solver_create1": (ci, [sz, sz, psz]),
+            "XAMG_solver_create2": (ci, [sz, sz, sz, sz, psz]),
+            "XAMG_solver_renew_params": (ci, [sz, sz]),
+            "XAMG_solver_solve1": (ci, [C.c_uint16, sz]),
+            "XAMG_solver_solve2": (ci, [C.c_uint16, sz, sz, sz]),
+            "XAMG_solver_print_info": (ci, [sz]),
+            "XAMG_solver_get_convergence_info": (ci, [vp, C.c_uint16, sz]),
+            "XAMG_solver_get_iter_info": (ci, [vp, sz]),
+            "XAMG_solver_destroy": (ci, [sz]),
+            "XAMG_timer": (cd, []),
+            "XAMG_barrier": (ci, []),
+            "xamgx_last_error": (cp, []),
+            "xamgx_device_info": (ci, [cp, ci]),
+            "xamgx_gen_poisson7_nnz": (C.c_int64, [ci, ci, ci]),
+            "xamgx_gen_poisson7": (ci, [ci, ci, ci, vp, vp, vp]),
+            "xamgx_gen_aniso27_nnz": (C.c_int64, [ci, ci, ci]),
+            "xamgx_gen_aniso27": (ci, [ci, ci, ci, cd, cd, cd, C.c_uint32, vp, vp, vp]),
+            "xamgx_gen_var5_nnz": (C.c_int64, [ci]),
+            "xamgx_gen_var5": (ci, [ci, C.c_uint32, vp, vp, vp]),
+            "xamgx_fill_rhs": (ci, [vp, C.c_int64, ci, C.c_uint32]),
+            "xamgx_spmv_host": (ci, [sz, ci, ci, ci, vp, vp]),
+            "xamgx_dot": (ci, [sz, sz, ci, vp]),
+            "xamgx_bench_spmv": (ci, [sz, ci, ci, ci, vp]),
+            "xamgx_bench_copy": (ci, [C.c_uint64, ci, vp]),
+            "xamgx_solver_last_ms": (ci, [sz, vp]),
+            "xamgx_solver_get_history": (ci, [sz, vp, ci, vp]),
+            "xamgx_solver_get_residuals": (ci, [sz, vp, vp]),
+            "xamgx_set_engine": (ci, [ci, ci, ci, ci, ci, ci]),
+            "xamgx_matrix_set_eigs": (ci, [sz, cd, cd]),
+            "xamgx_matrix_get_eigs": (ci, [sz, vp, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+def check(rc):
+    if rc != 0:
+        raise XamgError(lib().xamgx_last_error().decode())
+    return rc
+
+
+def _new(fn, *args):
+    h = C.c_size_t()
+    check(fn(C.byref(h), *args) if args else fn(C.byref(h)))
+    return h.value
+
+
+# ---------------------------------------------------------------------------
+# thin object layer over the handles (the reference's C API usage pattern,
+# examples/c/example.c:146-256)
+# ---------------------------------------------------------------------------
+
+def init():
+    check(lib().XAMG_init(0, None, b""))
+
+
+def finalize():
+    check(lib().XAMG_finalize())
+
+
+class Params:
+    def __init__(self, d=None, overrides=None):
+        self.id = _new(lib().XAMG_param_create)
+        for role, kv in (d or {}).items():
+            for k, v in kv.items():
+                self.add(role, k, v)
+        for role, kv in (overrides or {}).items():
+            ov = _new(lib().XAMG_param_override_create)
+            for k, v in kv.items():
+                check(lib().XAMG_param_override_add_value(_b(k), _b(v), ov))
+            check(lib().XAMG_param_override_apply(_b(role), self.id, ov))
+            check(lib().XAMG_param_override_destroy(ov))
+
+    def add(self, role, key, val):
+        L = lib()
+        if isinstance(val, bool):
+            check(L.XAMG_param_add_value_b(_b(role), _b(key), int(val), self.id))
+        elif isinstance(val, int):
+            check(L.XAMG_param_add_value_i(_b(role), _b(key), val, self.id))
+        elif isinstance(val, float):
+            check(L.XAMG_param_add_value_d(_b(role), _b(key), val, self.id))
+        else:
+            check(L.XAMG_param_add_value_s(_b(role), _b(key), _b(str(val)), self.id))
+
+    def change(self, role, key, val, forced=False):
+        L = lib()
+        pre = "XAMG_param_forced_change_value_" if forced else "XAMG_param_change_value_"
+        if isinstance(val, bool):
+            check(getattr(L, pre + "b")(_b(role), _b(key), int(val), self.id))
+        elif isinstance(val, int):
+            check(getattr(L, pre + "i")(_b(role), _b(key), val, self.id))
+        elif isinstance(val, float):
+            check(getattr(L, pre + "d")(_b(role), _b(key), val, self.id))
+        else:
+            check(getattr(L, pre + "s")(_b(role), _b(key), _b(str(val)), self.id))
+
+    def set_default(self):
+        lib().XAMG_param_set_default(self.id)
+
+    def destroy(self):
+        check(lib().XAMG_param_destroy(self.id))
+
+
+class Vector:
+    def __init__(self, n, nv, mode=MEM_DISTRIBUTED):
+        self.n, self.nv = n, nv
+        self.id = _new(lib().XAMG_vector_create, mode)
+        check(lib().XAMG_vector_alloc_d(n, nv, self.id))
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        assert a.size == self.n * self.nv
+        check(lib().XAMG_vector_upload_d(_p(a), self.n, 0, self.nv, self.id))
+
+    def download(self):
+        out = np.empty(self.n * self.nv, dtype=np.float64)
+        check(lib().XAMG_vector_download_d(_p(out), self.n, 0, self.nv, self.id))
+        return out
+
+    def set_val(self, v):
+        check(lib().XAMG_vector_set_val_d(float(v), self.id))
+
+    def destroy(self):
+        check(lib().XAMG_vector_destroy(self.id))
+
+
+class Matrix:
+    """Host CSR -> XAMG_csr_matrix_* -> XAMG_matrix_construct (upload to HBM)."""
+
+    def __init__(self, row, col, val, ncols=None):
+        L = lib()
+        row = np.ascontiguousarray(row, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        n = len(row) - 1
+        self.n, self.nnz = n, len(col)
+        csr = _new(L.XAMG_csr_matrix_create)
+        check(L.XAMG_csr_matrix_alloc(n, ncols if ncols is not None else n, len(col), csr))
+        check(L.XAMG_csr_matrix_fill(_p(row), _p(col), _p(val), csr))
+        check(L.XAMG_csr_matrix_offset(0, 0, csr))
+        self.part = _new(L.XAMG_part_create)
+        check(L.XAMG_part_construct(L.XAMG_csr_matrix_get_nrows(csr), self.part))
+        self.id = _new(L.XAMG_matrix_create)
+        check(L.XAMG_matrix_set_part(self.part, self.id))
+        check(L.XAMG_matrix_construct(csr, self.id))
+        check(L.XAMG_csr_matrix_destroy(csr))
+
+    def spmv(self, x, nv, prec=0, y=None, mode=0):
+        dt = np.float64 if prec == 0 else np.float32
+        x = np.ascontiguousarray(x, dtype=dt)
+        out = np.zeros(self.n * nv, dtype=dt) if y is None else np.ascontiguousarray(y, dtype=dt).copy()
+        check(lib().xamgx_spmv_host(self.id, prec, mode, nv, _p(x), _p(out)))
+        return out
+
+    def set_eigs(self, lo, hi):
+        check(lib().xamgx_matrix_set_eigs(self.id, lo, hi))
+
+    def get_eigs(self):
+        lo, hi = C.c_double(), C.c_double()
+        check(lib().xamgx_matrix_get_eigs(self.id, C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def bench_spmv(self, nv, prec=0, iters=100):
+        ms = C.c_double()
+        check(lib().xamgx_bench_spmv(self.id, prec, nv, iters, C.byref(ms)))
+        return ms.value
+
+    def destroy(self):
+        check(lib().XAMG_matrix_destroy(self.id))
+
+
+class Solver:
+    def __init__(self, matrix, params, x, b):
+        self.nv = x.nv
+        self.x, self.b = x, b
+        self.id = _new(lib().XAMG_solver_create2, matrix.id, params.id, x.id, b.id)
+
+    def solve(self, x=None, b=None):
+        if x is None:
+            check(lib().XAMG_solver_solve1(self.nv, self.id))
+        else:
+            check(lib().XAMG_solver_solve2(self.nv, x.id, b.id, self.id))
+
+    def renew(self, params):
+        check(lib().XAMG_solver_renew_params(params.id, self.id))
+
+    def iters(self):
+        a = np.zeros(2, dtype=np.int32)
+        check(lib().XAMG_solver_get_iter_info(_p(a), self.id))
+        return int(a[0]), int(a[1])
+
+    def converged(self):
+        a = np.zeros(self.nv, dtype=np.int32)
+        check(lib().XAMG_solver_get_convergence_info(_p(a), self.nv, self.id))
+        return [bool(v) for v in a]
+
+    def history(self, cap=100000):
+        out = np.zeros(cap * self.nv)
+        n = C.c_int()
+        check(lib().xamgx_solver_get_history(self.id, _p(out), cap, C.byref(n)))
+        k = min(n.value, cap)
+        return out[: k * self.nv].reshape(k, self.nv)
+
+    def residuals(self):
+        r0 = np.zeros(self.nv)
+        r = np.zeros(self.nv)
+        check(lib().xamgx_solver_get_residuals(self.id, _p(r0), _p(r)))
+        return r0, r
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().xamgx_solver_last_ms(self.id, C.byref(ms)))
+        return ms.value
+
+    def destroy(self):
+        check(lib().XAMG_solver_destroy(self.id))
+
+
+# ---------------------------------------------------------------------------
+# generators and helpers
+# ---------------------------------------------------------------------------
+
+def poisson7(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    nnz = lib().xamgx_gen_poisson7_nnz(nx, ny, nz)
+    row = np.empty(n + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    check(lib().xamgx_gen_poisson7(nx, ny, nz, _p(row), _p(col), _p(val)))
+    return row, col, val
+
+
+def aniso27(nx, ny, nz, k=(1.0, 1e-2, 1e-3), seed=3):
+    n = nx * ny * nz
+    nnz = lib().xamgx_gen_aniso27_nnz(nx, ny, nz)
+    row = np.empty(n + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    check(lib().xamgx_gen_aniso27(nx, ny, nz, k[0], k[1], k[2], seed, _p(row), _p(col), _p(val)))
+    return row, col, val
+
+
+def var5(s, seed):
+    n = s * s
+    nnz = lib().xamgx_gen_var5_nnz(s)
+    row = np.empty(n + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    check(lib().xamgx_gen_var5(s, seed, _p(row), _p(col), _p(val)))
+    return row, col, val
+
+
+def rhs(n, nv, seed):
+    b = np.empty(n * nv, dtype=np.float64)
+    check(lib().xamgx_fill_rhs(_p(b), n, nv, seed))
+    return b
+
+
+def set_engine(batch=-1, use_graphs=-1, grid_spmv=-1, grid_ew=-1, grid_red=-1, xcd_remap=-1):
+    check(lib().xamgx_set_engine(batch, use_graphs, grid_spmv, grid_ew, grid_red, xcd_remap))
+
+
+def bench_copy(nbytes, iters=20):
+    g = C.c_double()
+    check(lib().xamgx_bench_copy(nbytes, iters, C.byref(g)))
+    return g.value
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    check(lib().xamgx_device_info(buf, 256))
+    return buf.value.decode()
+
+
+def solve_system(row, col, val, b, nv, params, x0=None, eigs=None, keep=False):
+    """Upload A and b, build the solver from `params` (role -> {key: value}), solve.
+    Returns a dict with x, iterations, convergence flags, residuals, history, time."""
+    A = Matrix(row, col, val)
+    if eigs is not None:
+        A.set_eigs(*eigs)
+    n = A.n
+    x = Vector(n, nv)
+    bv = Vector(n, nv)
+    bv.upload(b)
+    x.upload(np.zeros(n * nv) if x0 is None else x0)
+    P = Params(params)
+    P.set_default()
+    S = Solver(A, P, x, bv)
+    S.solve()
+    it, sub = S.iters()
+    r0, r = S.residuals()
+    out = {"x": x.download(), "iters": it, "sub_iters": sub, "converged": S.converged(),
+           "res0": r0, "res": r, "hist": S.history(), "ms": S.last_ms()}
+    if keep:
+        out["objs"] = (A, x, bv, P, S)
+    else:
+        S.destroy()
+        P.destroy()
+        x.destroy()
+        bv.destroy()
+        A.destroy()
+    return out
