@@ -27,11 +27,15 @@ def poisson_eigs(n):
     return 1.0 - c, 1.0 + c
 
 
-def paus_system(n):
-    """Reference examples/test system (generate.hpp:188-286 + mtx_gen.cpp:9-153):
-    2D Poisson on (n+1)^2 points with identity boundary rows, b = cos(h i) sin(h j)
-    (times 2 h^2 in the interior), h = pi/(2n)."""
+def paus_system(n, scaled=False):
+    """Reference examples/test system: 2D Poisson on (n+1)^2 points with identity
+    boundary rows and b = cos(h i) sin(h j) (times 2 h^2 in the interior), h = pi/(2n).
+    scaled=False: interior stencil 4/-1 as generated in-process by generate.hpp:131-186
+    (the system behind out.yml); scaled=True: 4/h^2 and -1/h^2 as written to
+    mtx_Paus.bin by mtx_gen.cpp:62-92."""
     h = np.pi / n / 2
+    v4 = 4.0 / h / h if scaled else 4.0
+    vm = -1.0 / h / h if scaled else -1.0
     N = (n + 1) * (n + 1)
     rows, cols, vals = [], [], []
     for i in range(n + 1):
@@ -40,9 +44,8 @@ def paus_system(n):
             if i == 0 or j == 0 or i == n or j == n:
                 rows.append(r); cols.append(r); vals.append(1.0)
             else:
-                for (cc, v) in ((r - (n + 1), -1.0 / h / h), (r - 1, -1.0 / h / h),
-                                (r, 4.0 / h / h), (r + 1, -1.0 / h / h),
-                                (r + (n + 1), -1.0 / h / h)):
+                for (cc, v) in ((r - (n + 1), vm), (r - 1, vm), (r, v4), (r + 1, vm),
+                                (r + (n + 1), vm)):
                     rows.append(r); cols.append(cc); vals.append(v)
     A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
     A.sort_indices()
