@@ -69,6 +69,9 @@ int xamgx_solver_get_residuals(XAMG_solver_id sid, double *res0, double *res);
  * A negative value keeps the current setting. */
 int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int grid_red,
                      int xcd_remap);
+/* Right-hand sides per lane of the SpMV kernel for nv and element size es (8: FP64
+ * vectors, 4: FP32 vectors); 0 restores the default.  Used by the autotuner. */
+int xamgx_set_spmv_vpt(int nv, int es, int vpt);
 /* Spectrum bounds of D^{-1/2} A D^{-1/2} used by Chebyshev (the reference takes
  * them from hypre's setup, hypre_base.inl:369-375). */
 int xamgx_matrix_set_eigs(XAMG_matrix_id mid, double lo, double hi);

@@ -25,22 +25,47 @@ def _params(method, prec=None, tol=1e-10, max_iters=500, extra=None):
     return p
 
 
-def _compare(gpu, orc, nv, hist_rtol=1e-6):
+PERTURB_CHUNKS = (2, 3, 7, 16)
+
+
+def _hist_dev(a, b, nv):
+    """Per-check relative deviation of two residual histories (NaN where either run had
+    already converged that RHS)."""
+    k = min(len(a), len(b))
+    d = np.full((k, nv), np.nan)
+    for it in range(k):
+        for l in range(nv):
+            if a[it, l] >= 0 and b[it, l] >= 0:
+                d[it, l] = abs(a[it, l] - b[it, l]) / b[it, l]
+    return d
+
+
+def _compare(gpu, orc, nv, perturbed=None):
+    """Iterations within 2, same convergence flags, and a residual history that stays as
+    close to the oracle's as the oracle itself stays under a change of dot-product
+    summation order: `perturbed` holds the oracle rerun with 2/3/7/16-chunk dots (the
+    reference on ranks of that many cores).  Per check, the GPU deviation must not
+    exceed 10x the running maximum of those deviations (plus 1e-10)."""
     assert abs(gpu["iters"] - orc["iters"]) <= 2, (gpu["iters"], orc["iters"])
     assert gpu["converged"] == orc["converged"]
     hg, ho = gpu["hist"], orc["hist"]
-    k = min(len(hg), len(ho))
-    assert k >= 2
-    # compare while the residual is above 1e-7 of the initial one
+    assert min(len(hg), len(ho)) >= 2
+    dg = _hist_dev(hg, ho, nv)
+    if perturbed is None:
+        k = min(len(dg), 5)
+        assert np.nanmax(dg[:k]) <= 1e-8
+        return
+    runs = perturbed if isinstance(perturbed, list) else [perturbed]
+    k = min([len(dg)] + [len(r["hist"]) for r in runs])
+    dp = np.zeros((k, nv))
+    for r in runs:
+        dp = np.fmax(dp, np.nan_to_num(_hist_dev(r["hist"], ho, nv)[:k], nan=0.0))
+    dp = np.fmax.accumulate(dp, axis=0)
     for it in range(k):
         for l in range(nv):
-            if ho[it, l] < 0 or hg[it, l] < 0:
+            if np.isnan(dg[it, l]):
                 continue
-            if ho[it, l] < 1e-14 * ho[0, l]:
-                continue
-            rel = abs(hg[it, l] - ho[it, l]) / ho[it, l]
-            tol = hist_rtol * max(1.0, (ho[0, l] / ho[it, l]) ** 0.5 * 1e-6)
-            assert rel <= max(tol, 1e-3 if it > 0.6 * k else tol), (it, l, hg[it, l], ho[it, l])
+            assert dg[it, l] <= 10 * dp[it, l] + 1e-10, (it, l, dg[it, l], dp[it, l])
 
 
 CASES = [
@@ -63,7 +88,9 @@ def test_krylov_parity_16cube(xamg, method, prec, nv):
     eigs = poisson_eigs(nx)
     gpu = xamg.solve_system(row, col, val, b, nv, params, eigs=eigs)
     orc = O.solve(params, O.csr(row, col, val), b, nv, eigs=eigs, gs_blocks=16)
-    _compare(gpu, orc, nv)
+    per = [O.solve(params, O.csr(row, col, val), b, nv, eigs=eigs, gs_blocks=16, nchunks=c)
+           for c in PERTURB_CHUNKS]
+    _compare(gpu, orc, nv, per)
     # FP64 check of the final residual and error against a direct solve
     A = poisson7_scipy(nx)
     X = gpu["x"].reshape(n, nv)
@@ -71,7 +98,7 @@ def test_krylov_parity_16cube(xamg, method, prec, nv):
     R = B - A @ X
     for l in range(nv):
         assert np.linalg.norm(R[:, l]) <= 1.01e-10 * np.linalg.norm(B[:, l]) * 1.5
-    xs = spl.spsolve(A.tocsc(), B)
+    xs = spl.spsolve(A.tocsc(), B).reshape(n, nv)
     assert np.linalg.norm(X - xs) / np.linalg.norm(xs) < 1e-8
 
 
@@ -84,7 +111,8 @@ def test_cfg1_jacobi_32cube(xamg):
     params = _params("PBiCGStab", {"method": "Jacobi"}, tol=1e-8)
     gpu = xamg.solve_system(row, col, val, b, nv, params)
     orc = O.solve(params, O.csr(row, col, val), b, nv)
-    _compare(gpu, orc, nv)
+    per = [O.solve(params, O.csr(row, col, val), b, nv, nchunks=c) for c in PERTURB_CHUNKS]
+    _compare(gpu, orc, nv, per)
     assert gpu["converged"] == [True]
 
 
@@ -146,5 +174,5 @@ def test_ir_fp32_inner_fp64_outer(xamg, inner):
         assert np.linalg.norm(B[:, l] - A @ X[:, l]) <= 1e-10 * np.linalg.norm(B[:, l])
     orc = O.solve(params, O.csr(row, col, val), b, nv)
     assert abs(gpu["iters"] - orc["iters"]) <= 2
-    xs = spl.spsolve(A.tocsc(), B)
+    xs = spl.spsolve(A.tocsc(), B).reshape(n, nv)
     assert np.linalg.norm(X - xs) / np.linalg.norm(xs) < 1e-8

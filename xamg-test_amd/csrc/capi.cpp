@@ -808,9 +808,18 @@ int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int 
         if (grid_ew > 0)
             c.grid_ew = grid_ew;
         if (grid_red > 0)
-            c.grid_red = std::min(grid_red, 4096);
+            c.grid_red = std::min(grid_red, 8192);
         if (xcd_remap >= 0)
             c.xcd_remap = xcd_remap;
+    });
+}
+
+int xamgx_set_spmv_vpt(int nv, int es, int vpt) {
+    return guard([&] {
+        check_nv(nv);
+        XAMG_REQUIRE(es == 4 || es == 8, "element size must be 4 or 8");
+        XAMG_REQUIRE(vpt == 0 || (vpt <= nv && nv % vpt == 0), "vpt must divide nv");
+        spmv_vpt_choice(nv, es) = vpt;
     });
 }
 

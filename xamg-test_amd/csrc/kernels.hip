@@ -155,23 +155,26 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
     }
 }
 
-// Block reduction of ND per-RHS accumulators, store of the block partial, and the
-// last-block final pass.  Threads with equal (tid % NV) share an RHS index.
-template <typename T, int NV, int ND>
-__device__ void reduce_and_finish(double (&acc)[ND], const RedArgs &r, const FinArgs<T> &f) {
+// Block reduction of per-RHS accumulators, store of the block partial, and the
+// last-block final pass.  Each lane holds VPT consecutive RHS starting at
+// (tid % LR) * VPT with LR = NV / VPT; lanes with equal (tid % LR) share RHS indices.
+template <typename T, int NV, int ND, int VPT>
+__device__ void reduce_and_finish(double (&acc)[ND][VPT], const RedArgs &r, const FinArgs<T> &f) {
+    constexpr int LR = NV / VPT;
     __shared__ double lds[kBlock / 64][ND][NV];
-    __shared__ unsigned ticket;
-    __shared__ double sums[ND * NV];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int G = gridDim.x;
 #pragma unroll
     for (int k = 0; k < ND; k++) {
-        double s = acc[k];
 #pragma unroll
-        for (int off = 32; off >= NV; off >>= 1)
-            s += __shfl_xor(s, off);
-        if (lane < NV)
-            lds[wave][k][lane] = s;
+        for (int v = 0; v < VPT; v++) {
+            double s = acc[k][v];
+#pragma unroll
+            for (int off = 32; off >= LR; off >>= 1)
+                s += __shfl_xor(s, off);
+            if (lane < LR)
+                lds[wave][k][lane * VPT + v] = s;
+        }
     }
     __syncthreads();
     if (tid < ND * NV) {
@@ -182,21 +185,28 @@ __device__ void reduce_and_finish(double (&acc)[ND], const RedArgs &r, const Fin
             s += lds[w][k][l];
         r.ws.partials[(int64_t)tid * r.ws.max_blocks + blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0)
-        ticket = __hip_atomic_fetch_add(r.ws.counter, 1u, __ATOMIC_ACQ_REL,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (ticket != (unsigned)(G - 1))
+    (void)G;
+    (void)f;
+}
+
+// Final pass of a fused reduction, one workgroup launched right after the producing
+// kernel (the kernel boundary makes every workgroup's partials visible across XCDs
+// without per-workgroup L2 write-backs).  Wave w sums pairs p = w, w+16, ... over the
+// G workgroup partials in a fixed order, then one lane applies the scalar update.
+constexpr int kFinBlock = 1024;
+template <typename T, int NV, int ND>
+__global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, int G) {
+    if (r.halt != nullptr && *r.halt)
         return;
-    __threadfence();
-    // Final pass: wave w sums pairs p = w, w+4, ... over all blocks in a fixed order.
-    for (int p = wave; p < ND * NV; p += kBlock / 64) {
+    if (r.set_halt_if_done && r.cs != nullptr && r.cs->done)
+        return;
+    __shared__ double sums[ND * NV];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int p = wave; p < ND * NV; p += kFinBlock / 64) {
         const double *src = r.ws.partials + (int64_t)p * r.ws.max_blocks;
         double s = 0.0;
         for (int b = lane; b < G; b += 64)
-            s += __builtin_nontemporal_load(src + b);
+            s += src[b];
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1)
             s += __shfl_xor(s, off);
@@ -208,83 +218,204 @@ __device__ void reduce_and_finish(double (&acc)[ND], const RedArgs &r, const Fin
         for (int p = 0; p < ND * NV; p++)
             r.ws.result[p] = sums[p];
         apply_fin(f, sums, NV);
-        __hip_atomic_store(r.ws.counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
+    }
+}
+
+template <typename T, int NV>
+static void launch_finish(int nd, const RedArgs &r, const FinArgs<T> &f, int G, hipStream_t s) {
+    if (nd == 1)
+        hipLaunchKernelGGL((k_finish<T, NV, 1>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
+    else if (nd == 3)
+        hipLaunchKernelGGL((k_finish<T, NV, 3>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
+}
+
+// Loads / stores of VPT consecutive vector elements (16-byte accesses where possible).
+template <typename T, int VPT>
+__device__ __forceinline__ void ld_vec(const T *__restrict__ p, T (&v)[VPT]) {
+    if constexpr (sizeof(T) * VPT % 16 == 0) {
+        using V4 = uint4;
+#pragma unroll
+        for (int q = 0; q < (int)(sizeof(T) * VPT / 16); q++) {
+            V4 t = ((const V4 *)p)[q];
+            __builtin_memcpy(&v[q * 16 / sizeof(T)], &t, 16);
+        }
+    } else if constexpr (sizeof(T) * VPT == 8) {
+        uint2 t = *(const uint2 *)p;
+        __builtin_memcpy(&v[0], &t, 8);
+    } else {
+#pragma unroll
+        for (int q = 0; q < VPT; q++)
+            v[q] = p[q];
+    }
+}
+
+template <typename T, int VPT>
+__device__ __forceinline__ void st_vec(T *__restrict__ p, const T (&v)[VPT]) {
+    if constexpr (sizeof(T) * VPT % 16 == 0) {
+#pragma unroll
+        for (int q = 0; q < (int)(sizeof(T) * VPT / 16); q++) {
+            uint4 t;
+            __builtin_memcpy(&t, &v[q * 16 / sizeof(T)], 16);
+            ((uint4 *)p)[q] = t;
+        }
+    } else if constexpr (sizeof(T) * VPT == 8) {
+        uint2 t;
+        __builtin_memcpy(&t, &v[0], 8);
+        *(uint2 *)p = t;
+    } else {
+#pragma unroll
+        for (int q = 0; q < VPT; q++)
+            p[q] = v[q];
     }
 }
 
 // ---------------------------------------------------------------------------
 // CSR SpMV with fused epilogue and optional fused dot products.
-// One lane per (row, RHS); NV lanes of a row read the same val/col entry and
-// NV consecutive x values (one 8*NV-byte segment per neighbour).
+// LR = NV/VPT lanes per row; each lane keeps VPT right-hand sides in registers and
+// reads them with 16-byte loads.  The nonzeros of a row are walked in chunks of U:
+// all column indices, values and x segments of a chunk are loaded first, then added
+// in ascending column order (the reference's order, csr_matvec.inl:213-231).
 // ---------------------------------------------------------------------------
-template <typename V, typename T, int NV, int EPI, int ND>
+template <typename V, typename T, int NV, int VPT, int EPI, int ND>
 __global__ void __launch_bounds__(kBlock)
     k_spmv(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
     if (r.halt != nullptr && *r.halt)
         return;
+    constexpr int LR = NV / VPT;
+    constexpr int RPB = kBlock / LR;
+    constexpr int U = (16 / VPT) > 2 ? (16 / VPT) : 2;
     const V *__restrict__ val = (const V *)a.val;
     const uint32_t *__restrict__ row = a.row;
     const uint32_t *__restrict__ col = a.col;
     const T *__restrict__ x = a.x;
-    constexpr int RPB = kBlock / NV;
-    const int tid = threadIdx.x, lr = tid / NV, l = tid % NV;
+    const int tid = threadIdx.x, lr = tid / LR, sub = tid % LR;
+    const int l0 = sub * VPT;
     const int G = gridDim.x;
     const int L = logical_block(blockIdx.x, G, remap);
     const int64_t r0 = a.n * L / G, r1 = a.n * (L + 1) / G;
-    double dacc[ND > 0 ? ND : 1];
+    constexpr int NDX = ND > 0 ? ND : 1;
+    double dacc[NDX][VPT];
 #pragma unroll
-    for (int k = 0; k < (ND > 0 ? ND : 1); k++)
-        dacc[k] = 0.0;
+    for (int k = 0; k < NDX; k++)
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            dacc[k][v] = 0.0;
     for (int64_t base = r0; base < r1; base += RPB) {
         const int64_t i = base + lr;
         if (i < r1) {
-            const int64_t e = i * NV + l;
+            const int64_t e = i * NV + l0;
             const uint32_t j0 = row[i], j1 = row[i + 1];
-            T acc = (EPI == EPI_AXPY) ? a.y[e] : (T)0;
-            for (uint32_t j = j0; j < j1; ++j) {
-                const uint32_t c = col[j];
-                T xv = x[(int64_t)c * NV + l];
-                if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST)
-                    xv = xv * a.dcol[c];
-                acc = acc + xv * to_t<T>(val[j]);
-            }
-            T out;
-            if constexpr (EPI == EPI_AXY || EPI == EPI_AXPY) {
-                out = acc;
-            } else if constexpr (EPI == EPI_RESID) {
-                out = (T)1.0 * a.b[e] + (T)(-1.0) * acc;
-            } else if constexpr (EPI == EPI_JACOBI) {
-                out = a.xin[e] + (a.wcoef[l] * a.d[i]) * (a.b[e] - acc);
+            T acc[VPT];
+            if constexpr (EPI == EPI_AXPY) {
+                ld_vec<T, VPT>(a.y + e, acc);
             } else {
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    acc[v] = (T)0;
+            }
+            for (uint32_t j = j0; j < j1; j += U) {
+                uint32_t c[U];
+                T av[U];
+                T xv[U][VPT];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool ok = j + u < j1;
+                    c[u] = ok ? col[j + u] : col[j0];
+                    av[u] = ok ? to_t<T>(val[j + u]) : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    ld_vec<T, VPT>(x + (int64_t)c[u] * NV + l0, xv[u]);
+                if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const T dc = a.dcol[c[u]];
+#pragma unroll
+                        for (int v = 0; v < VPT; v++)
+                            xv[u][v] = xv[u][v] * dc;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j + u < j1)
+#pragma unroll
+                        for (int v = 0; v < VPT; v++)
+                            acc[v] = acc[v] + xv[u][v] * av[u];
+            }
+            T out[VPT];
+            if constexpr (EPI == EPI_AXY || EPI == EPI_AXPY) {
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    out[v] = acc[v];
+            } else if constexpr (EPI == EPI_RESID) {
+                T bb[VPT];
+                ld_vec<T, VPT>(a.b + e, bb);
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    out[v] = (T)1.0 * bb[v] + (T)(-1.0) * acc[v];
+            } else if constexpr (EPI == EPI_JACOBI) {
+                T bb[VPT], xi[VPT];
+                ld_vec<T, VPT>(a.b + e, bb);
+                ld_vec<T, VPT>(a.xin + e, xi);
                 const T di = a.d[i];
-                const T rr = a.b[e] * di;
-                T t2 = acc * di;
-                out = a.coef * rr + (T)1.0 * t2;
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    out[v] = xi[v] + (a.wcoef[l0 + v] * di) * (bb[v] - acc[v]);
+            } else {
+                T bb[VPT];
+                ld_vec<T, VPT>(a.b + e, bb);
+                const T di = a.d[i];
+#pragma unroll
+                for (int v = 0; v < VPT; v++) {
+                    const T rr = bb[v] * di;
+                    const T t2 = acc[v] * di;
+                    out[v] = a.coef * rr + (T)1.0 * t2;
+                }
                 if constexpr (EPI == EPI_CHEB_LAST) {
-                    out = out * di;
-                    if (a.xin != nullptr)
-                        out = (T)1.0 * out + (T)1.0 * a.xin[e];
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        out[v] = out[v] * di;
+                    if (a.xin != nullptr) {
+                        T xi[VPT];
+                        ld_vec<T, VPT>(a.xin + e, xi);
+#pragma unroll
+                        for (int v = 0; v < VPT; v++)
+                            out[v] = (T)1.0 * out[v] + (T)1.0 * xi[v];
+                    }
                 }
             }
-            a.y[e] = out;
+            st_vec<T, VPT>(a.y + e, out);
             if (a.y2 != nullptr)
-                a.y2[e] = out;
+                st_vec<T, VPT>(a.y2 + e, out);
             if (a.y3 != nullptr)
-                a.y3[e] = out;
+                st_vec<T, VPT>(a.y3 + e, out);
             if constexpr (ND == 1) {
-                const double wv = a.w != nullptr ? (double)a.w[e] : (double)out;
-                dacc[0] += (double)out * wv;
+                if (a.w != nullptr) {
+                    T ww[VPT];
+                    ld_vec<T, VPT>(a.w + e, ww);
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        dacc[0][v] += (double)out[v] * (double)ww[v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        dacc[0][v] += (double)out[v] * (double)out[v];
+                }
             } else if constexpr (ND == 3) {
-                const double wv = (double)a.w[e];
-                dacc[0] += (double)out * wv;
-                dacc[1] += (double)out * (double)out;
-                dacc[2] += wv * wv;
+                T ww[VPT];
+                ld_vec<T, VPT>(a.w + e, ww);
+#pragma unroll
+                for (int v = 0; v < VPT; v++) {
+                    const double o = (double)out[v], wv = (double)ww[v];
+                    dacc[0][v] += o * wv;
+                    dacc[1][v] += o * o;
+                    dacc[2][v] += wv * wv;
+                }
             }
         }
     }
     if constexpr (ND > 0)
-        reduce_and_finish<T, NV, ND>(dacc, r, f);
+        reduce_and_finish<T, NV, ND, VPT>(dacc, r, f);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,10 +440,10 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
     const T ca = a.a != nullptr ? a.a[l] : (T)1.0;
     const T cb = a.b != nullptr ? a.b[l] : (T)1.0;
     const T cc = a.c != nullptr ? a.c[l] : (T)1.0;
-    double dacc[ND > 0 ? ND : 1];
+    double dacc[ND > 0 ? ND : 1][1];
 #pragma unroll
     for (int k = 0; k < (ND > 0 ? ND : 1); k++)
-        dacc[k] = 0.0;
+        dacc[k][0] = 0.0;
     for (int64_t e = e0 + tid; e < e1; e += kBlock) {
         T out;
         if constexpr (OP == EW_FILL) {
@@ -352,11 +483,11 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
         a.z[e] = out;
         if constexpr (ND == 1) {
             const double wv = a.w != nullptr ? (double)a.w[e] : (double)out;
-            dacc[0] += (double)out * wv;
+            dacc[0][0] += (double)out * wv;
         }
     }
     if constexpr (ND > 0)
-        reduce_and_finish<T, NV, ND>(dacc, r, f);
+        reduce_and_finish<T, NV, ND, 1>(dacc, r, f);
 }
 
 // Dense GEMV for the coarse Direct solve (dense_matvec.inl:100-117): one lane per
@@ -383,7 +514,7 @@ __global__ void __launch_bounds__(kBlock)
 // RHS in parallel while the rows of the block are visited in order (forward then
 // backward), exactly the sequential loop of csr_gs.inl:77-130 per block.
 template <typename V, typename T, int NV>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kBlock)
     k_hsgs_offd(const uint32_t *__restrict__ row, const uint32_t *__restrict__ col,
                 const V *__restrict__ val, int64_t n, const T *__restrict__ b,
                 const T *__restrict__ x, T *__restrict__ t, int nblocks, int64_t bsize,
@@ -446,20 +577,22 @@ __global__ void __launch_bounds__(64)
 // Host launchers
 // ---------------------------------------------------------------------------
 
-template <typename V, typename T, int NV>
-static void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
-                    hipStream_t s) {
+template <typename V, typename T, int NV, int VPT>
+static void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
+                     hipStream_t s) {
     const LaunchCfg &c = launch_cfg();
     int G = nd > 0 ? c.grid_red : c.grid_spmv;
     if (nd > 0 && G > r.ws.max_blocks)
         G = r.ws.max_blocks;
-    int64_t need = (a.n * NV + kBlock - 1) / kBlock;
+    constexpr int LR = NV / VPT;
+    int64_t need = (a.n * LR + kBlock - 1) / kBlock;
     if (need < G)
         G = (int)(need > 0 ? need : 1);
     dim3 g(G), b(kBlock);
 #define XAMG_SPMV_CASE(E, D)                                                                       \
     if (epi == E && nd == D) {                                                                     \
-        hipLaunchKernelGGL((k_spmv<V, T, NV, E, D>), g, b, 0, s, a, r, f, c.xcd_remap);            \
+        hipLaunchKernelGGL((k_spmv<V, T, NV, VPT, E, D>), g, b, 0, s, a, r, f, c.xcd_remap);       \
+        launch_finish<T, NV>(D, r, f, G, s);                                                       \
         return;                                                                                    \
     }
     XAMG_SPMV_CASE(EPI_AXY, 0)
@@ -473,6 +606,45 @@ static void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, con
     XAMG_SPMV_CASE(EPI_CHEB_LAST, 0)
 #undef XAMG_SPMV_CASE
     throw std::runtime_error("xamg: unsupported spmv epilogue/dot combination");
+}
+
+// RHS-per-lane choice (tunable): index into the compiled widths of each nv.
+int &spmv_vpt_choice(int nv, int es) {
+    static int tab[2][17] = {};
+    return tab[es == 8 ? 0 : 1][nv];
+}
+
+template <typename V, typename T, int NV>
+static void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
+                    hipStream_t s) {
+    int vpt = spmv_vpt_choice(NV, (int)sizeof(T));
+    if constexpr (NV == 1) {
+        spmv_nvv<V, T, 1, 1>(epi, nd, a, r, f, s);
+    } else if constexpr (NV == 2) {
+        if (vpt == 1)
+            spmv_nvv<V, T, 2, 1>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 2, 2>(epi, nd, a, r, f, s);
+    } else if constexpr (NV == 4) {
+        if (vpt == 2)
+            spmv_nvv<V, T, 4, 2>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 4, 4>(epi, nd, a, r, f, s);
+    } else if constexpr (NV == 8) {
+        if (vpt == 2)
+            spmv_nvv<V, T, 8, 2>(epi, nd, a, r, f, s);
+        else if (vpt == 8)
+            spmv_nvv<V, T, 8, 8>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 8, 4>(epi, nd, a, r, f, s);
+    } else {
+        if (vpt == 8)
+            spmv_nvv<V, T, 16, 8>(epi, nd, a, r, f, s);
+        else if (vpt == 16)
+            spmv_nvv<V, T, 16, 16>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 16, 4>(epi, nd, a, r, f, s);
+    }
 }
 
 template <typename V, typename T>
@@ -514,6 +686,7 @@ static void ew_nv(int op, int nd, const EwArgs<T> &a, const RedArgs &r, const Fi
 #define XAMG_EW_CASE(O, D)                                                                         \
     if (op == O && nd == D) {                                                                      \
         hipLaunchKernelGGL((k_ew<T, NV, O, D>), g, b, 0, s, a, r, f, c.xcd_remap);                 \
+        launch_finish<T, NV>(D, r, f, G, s);                                                       \
         return;                                                                                    \
     }
     XAMG_EW_CASE(EW_FILL, 0)
@@ -672,17 +845,28 @@ namespace xamg {
 
 // Streaming copy (16-byte lanes) used to measure the attainable HBM bandwidth that the
 // SpMV roofline fraction is quoted against.
+// Each thread moves 4 x 16 bytes: the four loads are issued before the stores.
 __global__ void __launch_bounds__(kBlock) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                  int64_t n16) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n16; i += stride)
-        dst[i] = src[i];
+    const int64_t i0 = ((int64_t)blockIdx.x * kBlock) * 4 + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int64_t i = i0 + (int64_t)q * kBlock;
+        if (i < n16)
+            v[q] = src[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int64_t i = i0 + (int64_t)q * kBlock;
+        if (i < n16)
+            dst[i] = v[q];
+    }
 }
 
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s) {
     const int64_t n16 = bytes / 16;
-    int grid = (int)std::min<int64_t>((n16 + kBlock - 1) / kBlock, 256 * 32);
+    int64_t grid = (n16 + 4 * kBlock - 1) / (4 * kBlock);
     hipLaunchKernelGGL(k_copy, dim3(grid > 0 ? grid : 1), dim3(kBlock), 0, s, (const uint4 *)src,
                        (uint4 *)dst, n16);
     XAMG_HIP_CHECK(hipGetLastError());

@@ -5,9 +5,11 @@
 // over 8 XCDs, each with its own L2): the workgroups of one XCD cover one contiguous
 // eighth of the rows, so the gathered neighbours of a stencil stay in that XCD's L2.
 //
-// Reductions are deterministic: per-workgroup partials in fixed lane/wave order, then
-// the last workgroup to finish (ticket counter) sums the partials in workgroup order
-// and applies the solver's scalar update ("final op") on the device.
+// Reductions are deterministic: per-workgroup partials in fixed lane/wave order, then a
+// one-workgroup finish kernel sums the partials in workgroup order and applies the
+// solver's scalar update ("final op") on the device.  (A last-workgroup ticket scheme
+// would need an agent-scope release per workgroup, i.e. an L2 write-back per workgroup
+// on a multi-XCD GPU; the kernel boundary provides the visibility once.)
 #pragma once
 
 #include "common.hpp"
@@ -104,11 +106,16 @@ struct EwArgs {
 struct LaunchCfg {
     int grid_spmv = 2048;
     int grid_ew = 2048;
-    int grid_red = 1024;
+    int grid_red = 2048;
     int xcd_remap = 1;
 };
 
 LaunchCfg &launch_cfg();
+
+// RHS per lane of the SpMV kernel for (nv, element size); 0 selects the default
+// (1 for nv=1, 2 for nv=2, 4 otherwise).  Compiled widths: nv=2 {1,2}, nv=4 {2,4},
+// nv=8 {2,4,8}, nv=16 {4,8,16}.
+int &spmv_vpt_choice(int nv, int es);
 
 // V value type: double, float or __hip_bfloat16; T vector type: double or float.
 template <typename V, typename T>

@@ -20,7 +20,7 @@ void Runtime::init() {
     XAMG_HIP_CHECK(hipGetDevice(&device));
     // default-priority stream (no priority changes, see the box notes on stream priorities)
     XAMG_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    const int max_blocks = 4096;
+    const int max_blocks = 8192;
     partials.alloc(sizeof(double) * kMaxDots * kMaxNV * max_blocks);
     counter.alloc(256);
     result.alloc(sizeof(double) * kMaxDots * kMaxNV);

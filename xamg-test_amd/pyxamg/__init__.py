@@ -108,6 +108,7 @@ def lib():
             "xamgx_solver_get_history": (ci, [sz, vp, ci, vp]),
             "xamgx_solver_get_residuals": (ci, [sz, vp, vp]),
             "xamgx_set_engine": (ci, [ci, ci, ci, ci, ci, ci]),
+            "xamgx_set_spmv_vpt": (ci, [ci, ci, ci]),
             "xamgx_matrix_set_eigs": (ci, [sz, cd, cd]),
             "xamgx_matrix_get_eigs": (ci, [sz, vp, vp]),
         }
@@ -267,7 +268,9 @@ class Solver:
     def __init__(self, matrix, params, x, b):
         self.nv = x.nv
         self.x, self.b = x, b
-        self.id = _new(lib().XAMG_solver_create2, matrix.id, params.id, x.id, b.id)
+        h = C.c_size_t()
+        check(lib().XAMG_solver_create2(matrix.id, params.id, x.id, b.id, C.byref(h)))
+        self.id = h.value
 
     def solve(self, x=None, b=None):
         if x is None:
@@ -354,6 +357,10 @@ def rhs(n, nv, seed):
 
 def set_engine(batch=-1, use_graphs=-1, grid_spmv=-1, grid_ew=-1, grid_red=-1, xcd_remap=-1):
     check(lib().xamgx_set_engine(batch, use_graphs, grid_spmv, grid_ew, grid_red, xcd_remap))
+
+
+def set_spmv_vpt(nv, es, vpt):
+    check(lib().xamgx_set_spmv_vpt(nv, es, vpt))
 
 
 def bench_copy(nbytes, iters=20):
