@@ -33,7 +33,7 @@ def spmv_bytes(n, nnz, nv, s_val, s_vec):
     return 4 * (n + 1) + nnz * (4 + s_val) + 2 * n * nv * s_vec
 
 
-def solver_params(kind, mixed, inner_rtol, tol):
+def solver_params(kind, mixed, inner_rtol, tol, refine="continued", step=1e-2):
     if kind == "cheb4":
         prec = {"method": "Chebyshev", "polynomial_order": 4, "spectrum_fraction": 0.3}
         method = "PBiCGStab"
@@ -49,7 +49,8 @@ def solver_params(kind, mixed, inner_rtol, tol):
     else:
         p = {"meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
                              "max_iters": 50, "rel_tolerance": tol, "abs_tolerance": 0.0,
-                             "convergence_details": "check-L2"},
+                             "convergence_details": "check-L2", "refinement": refine,
+                             "replace_step": step},
              "solver": {"method": method, "rel_tolerance": inner_rtol, "abs_tolerance": 0.0,
                         "max_iters": 5000, "convergence_details": "check-L2"}}
     if prec is not None:
@@ -94,7 +95,10 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--nv", type=int, default=8)
     ap.add_argument("--solver", default="cheb4", choices=["cheb4", "jacobi", "bicgstab"])
-    ap.add_argument("--inner-rtol", type=float, default=1e-5)
+    ap.add_argument("--inner-rtol", type=float, default=1e-4,
+                    help="inner tolerance of the restarted refinement (reference IR)")
+    ap.add_argument("--refine", default="continued", choices=["continued", "restart"])
+    ap.add_argument("--replace-step", type=float, default=1e-2)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -124,7 +128,8 @@ def main():
     bv.upload(b)
 
     def make(mixed):
-        P = X.Params(solver_params(args.solver, mixed, args.inner_rtol, args.tol))
+        P = X.Params(solver_params(args.solver, mixed, args.inner_rtol, args.tol, args.refine,
+                                   args.replace_step))
         P.set_default()
         return X.Solver(A, P, xv, bv)
 
@@ -189,6 +194,7 @@ def main():
         "data": "synthetic (7-point Poisson generator, pseudo_rand RHS seed 2)",
         "config": {"workload": f"7pt-poisson {n1}^3 nv={nv} {args.solver} rtol={args.tol}",
                    "rows": n, "nnz": nnz, "nv": nv, "solver": args.solver,
+                   "refinement": args.refine, "replace_step": args.replace_step,
                    "inner_rel_tolerance": args.inner_rtol, "parallelism": "1 GPU"},
         "fp64_time_s": mean_64,
         "speedup_vs_fp64": mean_64 / mean_mx,

@@ -176,3 +176,43 @@ def test_ir_fp32_inner_fp64_outer(xamg, inner):
     assert abs(gpu["iters"] - orc["iters"]) <= 2
     xs = spl.spsolve(A.tocsc(), B).reshape(n, nv)
     assert np.linalg.norm(X - xs) / np.linalg.norm(xs) < 1e-8
+
+
+@pytest.mark.parametrize("prec", [None, {"method": "Jacobi"},
+                                  {"method": "Chebyshev", "polynomial_order": 4}])
+def test_continued_refinement_fp32_krylov_fp64_residual(xamg, prec):
+    """IterativeRefinement with refinement=continued: FP32 (P)BiCGStab recurrences with
+    FP64 residual replacement.  No reference counterpart (the reference restarts the
+    inner solver): pinned by the FP64-verified residual, the error against a direct FP64
+    solution, and an FP32 iteration count close to the FP64 oracle's."""
+    nx, nv = 24, 4
+    row, col, val = xamg.poisson7(nx)
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 2)
+    method = "PBiCGStab" if prec else "BiCGStab"
+    params = {
+        "meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
+                        "refinement": "continued", "rel_tolerance": 1e-10,
+                        "abs_tolerance": 0.0, "convergence_details": "check-L2"},
+        "solver": {"method": method, "max_iters": 1000},
+    }
+    if prec:
+        params["preconditioner"] = dict(prec)
+    eigs = poisson_eigs(nx)
+    gpu = xamg.solve_system(row, col, val, b, nv, params, eigs=eigs)
+    assert all(gpu["converged"])
+    A = poisson7_scipy(nx)
+    X = gpu["x"].reshape(n, nv)
+    B = b.reshape(n, nv)
+    for l in range(nv):
+        assert np.linalg.norm(B[:, l] - A @ X[:, l]) <= 1e-10 * np.linalg.norm(B[:, l])
+    xs = spl.spsolve(A.tocsc(), B).reshape(n, nv)
+    assert np.linalg.norm(X - xs) / np.linalg.norm(xs) < 1e-8
+    p64 = {"solver": {"method": method, "rel_tolerance": 1e-10, "abs_tolerance": 0.0,
+                      "max_iters": 1000}}
+    if prec:
+        p64["preconditioner"] = dict(prec)
+    orc = O.solve(p64, O.csr(row, col, val), b, nv, eigs=eigs)
+    outer, inner = gpu["iters"], gpu["sub_iters"]
+    assert 1 <= outer <= 20
+    assert inner <= 1.3 * orc["iters"] + 5, (inner, orc["iters"])

@@ -55,7 +55,10 @@ struct ConvState {
     int halt;      // loop kernels return immediately once set
     int nhist;     // residual-history rows written
     int hist_cap;
-    int pad[2];
+    // continued refinement (FP32 Krylov, FP64 residual replacement)
+    int replace;            // set when the FP64 true residual must be recomputed
+    int replaced;           // set by that recomputation for the following r update
+    double maxres[kMaxNV];  // squared residual at the last FP64 check
 };
 
 // Krylov scalars of one BiCGStab/PBiCGStab instance, precision of the iteration.

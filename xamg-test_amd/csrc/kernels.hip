@@ -150,9 +150,79 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
             cs->halt = cs->done;
         }
         break;
+    case FIN_MX_INIT: {
+        auto *k32 = (KrylovScalars<float> *)f.aux;
+        for (int l = 0; l < nv; l++) {
+            v[l] = (F)sums[l];
+            k32->rho[l] = (float)sums[l];
+            cs->maxres[l] = sums[l];
+        }
+        cs->replace = cs->replaced = 0;
+        conv_check(f, v, nv, 1);
+        cs->halt = cs->done;
+        if (!cs->done)
+            for (int l = 0; l < nv; l++)
+                if (cs->status[l])
+                    k32->iconv[l] = 0.0f;
+        break;
+    }
+    case FIN_MX_OMEGA: {
+        int req = 0;
+        for (int l = 0; l < nv; l++) {
+            F s2 = (F)sums[l], s3 = (F)sums[nv + l], s4 = (F)sums[2 * nv + l];
+            F om = s2 / s3;
+            ks->omega[l] = om;
+            ks->nomega[l] = om * (F)-1.0;
+            ks->ac[l] = ks->iconv[l] * ks->alpha[l];
+            ks->oc[l] = ks->iconv[l] * om;
+            F res = om * s2;
+            res = (F)1.0 * s4 + (F)(-1.0) * res;
+            v[l] = res;
+            if (!cs->status[l]) {
+                const double rr = (double)res;
+                cs->res[l] = rr;
+                if (rr <= cs->tol[l] * cs->tol[l] || rr <= f.delta2 * cs->maxres[l])
+                    req = 1;
+            }
+        }
+        if (f.hist != nullptr && cs->nhist < cs->hist_cap) {
+            for (int l = 0; l < nv; l++)
+                f.hist[(int64_t)cs->nhist * nv + l] = cs->status[l] ? -1.0 : (double)v[l];
+            cs->nhist++;
+        }
+        if (req)
+            cs->replace = 1;
+        break;
+    }
+    case FIN_MX_REPL: {
+        auto *k32 = (KrylovScalars<float> *)f.aux;
+        cs->sub_iter++;
+        int flag = 1;
+        for (int l = 0; l < nv; l++) {
+            if (cs->status[l])
+                continue;
+            cs->res[l] = sums[l];
+            cs->maxres[l] = sums[l];
+            if (sqrt(sums[l]) <= cs->tol[l])
+                cs->status[l] = 1;
+            else
+                flag = 0;
+        }
+        if (flag)
+            cs->done = 1;
+        else
+            for (int l = 0; l < nv; l++)
+                if (cs->status[l])
+                    k32->iconv[l] = 0.0f;
+        cs->replace = 0;
+        cs->replaced = 1;
+        break;
+    }
     default:
         break;
     }
+    if (f.clear_on_fin != nullptr)
+        *f.clear_on_fin = 0;
 }
 
 // Block reduction of per-RHS accumulators, store of the block partial, and the
@@ -198,15 +268,28 @@ template <typename T, int NV, int ND>
 __global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, int G) {
     if (r.halt != nullptr && *r.halt)
         return;
+    if (r.gate != nullptr && !*r.gate)
+        return;
     if (r.set_halt_if_done && r.cs != nullptr && r.cs->done)
         return;
     __shared__ double sums[ND * NV];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int p = wave; p < ND * NV; p += kFinBlock / 64) {
         const double *src = r.ws.partials + (int64_t)p * r.ws.max_blocks;
-        double s = 0.0;
-        for (int b = lane; b < G; b += 64)
-            s += src[b];
+        // 8 independent loads in flight per lane; fixed combination order
+        double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int b0 = lane; b0 < G; b0 += 64 * 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int b = b0 + 64 * u;
+                v[u] = b < G ? src[b] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                q[u] += v[u];
+        }
+        double s = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1)
             s += __shfl_xor(s, off);
@@ -281,6 +364,8 @@ __global__ void __launch_bounds__(kBlock)
     k_spmv(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
     if (r.halt != nullptr && *r.halt)
         return;
+    if (r.gate != nullptr && !*r.gate)
+        return;
     constexpr int LR = NV / VPT;
     constexpr int RPB = kBlock / LR;
     constexpr int U = (16 / VPT) > 2 ? (16 / VPT) : 2;
@@ -327,12 +412,24 @@ __global__ void __launch_bounds__(kBlock)
                 for (int u = 0; u < U; u++)
                     ld_vec<T, VPT>(x + (int64_t)c[u] * NV + l0, xv[u]);
                 if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
+                    if (a.x_is_rhs) {
+                        // first Horner step read straight from the right-hand side:
+                        // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
 #pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const T dc = a.dcol[c[u]];
+                        for (int u = 0; u < U; u++) {
+                            const T dc = a.dcol[c[u]];
 #pragma unroll
-                        for (int v = 0; v < VPT; v++)
-                            xv[u][v] = xv[u][v] * dc;
+                            for (int v = 0; v < VPT; v++)
+                                xv[u][v] = (a.xcoef * (xv[u][v] * dc)) * dc;
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const T dc = a.dcol[c[u]];
+#pragma unroll
+                            for (int v = 0; v < VPT; v++)
+                                xv[u][v] = xv[u][v] * dc;
+                        }
                     }
                 }
 #pragma unroll
@@ -347,7 +444,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
                 for (int v = 0; v < VPT; v++)
                     out[v] = acc[v];
-            } else if constexpr (EPI == EPI_RESID) {
+            } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
                 T bb[VPT];
                 ld_vec<T, VPT>(a.b + e, bb);
 #pragma unroll
@@ -384,11 +481,23 @@ __global__ void __launch_bounds__(kBlock)
                     }
                 }
             }
-            st_vec<T, VPT>(a.y + e, out);
-            if (a.y2 != nullptr)
-                st_vec<T, VPT>(a.y2 + e, out);
-            if (a.y3 != nullptr)
-                st_vec<T, VPT>(a.y3 + e, out);
+            if constexpr (EPI == EPI_RESID_MX) {
+                float o32[VPT];
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    o32[v] = (float)out[v];
+                st_vec<float, VPT>(a.yo + e, o32);
+                if (a.yo2 != nullptr)
+                    st_vec<float, VPT>(a.yo2 + e, o32);
+                if (a.yo3 != nullptr)
+                    st_vec<float, VPT>(a.yo3 + e, o32);
+            } else {
+                st_vec<T, VPT>(a.y + e, out);
+                if (a.y2 != nullptr)
+                    st_vec<T, VPT>(a.y2 + e, out);
+                if (a.y3 != nullptr)
+                    st_vec<T, VPT>(a.y3 + e, out);
+            }
             if constexpr (ND == 1) {
                 if (a.w != nullptr) {
                     T ww[VPT];
@@ -431,6 +540,9 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
             r.cs->halt = 1;
         return;
     }
+    if (r.gate != nullptr && !*r.gate)
+        return;
+    const bool keep = a.keep_if != nullptr && *a.keep_if;
     const int tid = threadIdx.x;
     const int l = tid % NV;
     const int G = gridDim.x;
@@ -449,7 +561,7 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
         if constexpr (OP == EW_FILL) {
             out = a.c0;
         } else if constexpr (OP == EW_AXPBY_Z) {
-            out = ca * a.x[e] + cb * a.y[e];
+            out = keep ? a.z[e] : ca * a.x[e] + cb * a.y[e];
         } else if constexpr (OP == EW_AXPBYPCZ) {
             out = ca * a.x[e] + cb * a.y[e] + cc * a.z[e];
         } else if constexpr (OP == EW_JACOBI0) {
@@ -477,8 +589,11 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
             out = (T)((const O *)a.xo)[e];
         } else if constexpr (OP == EW_SCAL_D) {
             out = a.z[e] * a.d[e / NV];
-        } else { // EW_JAC_UPD
+        } else if constexpr (OP == EW_JAC_UPD) {
             out = (ca * cc) * (a.y[e] * a.d[e / NV]) + (T)1.0 * a.z[e];
+        } else { // EW_MX_XUPD: FP64 accumulation of FP32 directions
+            out = (T)a.ao[l] * (T)((const float *)a.xo)[e] +
+                  (T)a.bo[l] * (T)((const float *)a.yo)[e] + (T)1.0 * a.z[e];
         }
         a.z[e] = out;
         if constexpr (ND == 1) {
@@ -604,6 +719,8 @@ static void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, co
     XAMG_SPMV_CASE(EPI_JACOBI, 0)
     XAMG_SPMV_CASE(EPI_CHEB, 0)
     XAMG_SPMV_CASE(EPI_CHEB_LAST, 0)
+    if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value)
+        XAMG_SPMV_CASE(EPI_RESID_MX, 1)
 #undef XAMG_SPMV_CASE
     throw std::runtime_error("xamg: unsupported spmv epilogue/dot combination");
 }
@@ -702,6 +819,8 @@ static void ew_nv(int op, int nd, const EwArgs<T> &a, const RedArgs &r, const Fi
     XAMG_EW_CASE(EW_SCAL_D, 0)
     XAMG_EW_CASE(EW_COPY3, 1)
     XAMG_EW_CASE(EW_JAC_UPD, 0)
+    if constexpr (std::is_same<T, double>::value)
+        XAMG_EW_CASE(EW_MX_XUPD, 0)
 #undef XAMG_EW_CASE
     throw std::runtime_error("xamg: unsupported elementwise op/dot combination");
 }

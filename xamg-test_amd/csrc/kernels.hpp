@@ -23,7 +23,8 @@ enum Epi : int {
     EPI_RESID = 2,    // y = 1*b + (-1)*acc                           (solver.h:183-190)
     EPI_JACOBI = 3,   // y = xin + (w*d_i)*(b - acc)                  (blas_merged_jacobi.inl:80-85)
     EPI_CHEB = 4,     // y = c*(b*d_i) + 1*(acc*d_i), x pre-scaled by d_col (chebyshev.inl:172-178)
-    EPI_CHEB_LAST = 5 // as EPI_CHEB then y = 1*(y*d_i) [+ 1*xold]   (chebyshev.inl:180-181)
+    EPI_CHEB_LAST = 5, // as EPI_CHEB then y = 1*(y*d_i) [+ 1*xold]  (chebyshev.inl:180-181)
+    EPI_RESID_MX = 6   // FP64 r = b - acc stored as FP32 (yo, yo2, yo3); dot in FP64
 };
 
 // Elementwise ops on [n][nv] vectors with per-RHS coefficients (nullptr = 1).
@@ -39,6 +40,7 @@ enum Ew : int {
     EW_CONVERT = 8,    // z(T) = (T)x(other precision)                (convert.inl:31-83)
     EW_SCAL_D = 9,     // z = z * d_i  (row scaling)                  (blas.inl:173-183)
     EW_JAC_UPD = 10,   // z = (a*c)*(y*d_i) + 1*z  (check-mode Jacobi, jacobi.inl:80-84)
+    EW_MX_XUPD = 11,   // z(FP64) = ao*(T)xo + bo*(T)yo + 1*z, FP32 directions/coefficients
 };
 
 // Scalar update applied by the last workgroup after a fused reduction.
@@ -50,7 +52,12 @@ enum Fin : int {
     FIN_BETA = 4,   // beta, rho update, -(omega*beta)                     (:176-199)
     FIN_NORM_INIT = 5, // res0 = ||r||^2, tolerance, check (Jacobi/MG/IR/Cheb check mode)
     FIN_NORM = 6,      // iter++ then check ||r||^2 against the tolerance
-    FIN_NORM_NOINC = 7 // check ||r||^2 without counting an iteration
+    FIN_NORM_NOINC = 7, // check ||r||^2 without counting an iteration
+    // continued refinement: FP32 Krylov recurrences with FP64 residual replacement
+    FIN_MX_INIT = 8,  // FP64 ||r0||^2: res0, tolerance, initial check; FP32 rho (aux)
+    FIN_MX_OMEGA = 9, // as FIN_OMEGA, but a recurrence residual below the tolerance or
+                      // below delta^2 * (last FP64 residual) requests an FP64 check
+    FIN_MX_REPL = 10  // FP64 true residual: per-RHS convergence, masks (aux), restart ref
 };
 
 template <typename F>
@@ -62,11 +69,15 @@ struct FinArgs {
     int conv_check = 0;
     double abs_tol = 0, rel_tol = 0;
     F *iconv = nullptr; // mask updated on partial convergence (nullptr: no mask)
+    void *aux = nullptr;         // FIN_MX_*: the FP32 KrylovScalars of the iteration
+    double delta2 = 0;           // FIN_MX_OMEGA: replacement step (squared)
+    int *clear_on_fin = nullptr; // flag reset to 0 by the final op
 };
 
 struct RedArgs {
     ReduceWs ws;
     const int *halt = nullptr; // skip the launch's work once set (loop early exit)
+    const int *gate = nullptr; // run only while *gate != 0 (conditional steps)
     int set_halt_if_done = 0;  // kernel sets *halt when the solver's done flag is up
     ConvState *cs = nullptr;
 };
@@ -85,9 +96,13 @@ struct SpmvArgs {
     const T *d = nullptr;    // row scaling (inv_diag / inv_sqrt_diag)
     const T *dcol = nullptr; // CHEB: column pre-scaling of x
     T coef = 0;              // CHEB coefficient c_q
+    int x_is_rhs = 0;        // CHEB: x is the (unscaled) RHS of the first Horner step
+    T xcoef = 0;             // CHEB: leading coefficient c_{k-1} applied to that RHS
     const T *wcoef = nullptr; // JACOBI per-RHS relax factors
     const T *w = nullptr;    // second operand of the fused dot(s)
     T *y2 = nullptr, *y3 = nullptr; // extra copies of y (initial residual: r, p)
+    // EPI_RESID_MX: FP32 copies of the FP64 residual (y is not written)
+    float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr;
 };
 
 template <typename T>
@@ -101,6 +116,9 @@ struct EwArgs {
     const void *xo = nullptr;                         // other-precision input (CONVERT/ADD_CONV)
     const T *w = nullptr;                             // fused dot: dot(z, w)
     T *y2 = nullptr, *y3 = nullptr;                   // COPY3 extra outputs
+    const void *yo = nullptr;                         // second other-precision input
+    const float *ao = nullptr, *bo = nullptr;         // FP32 coefficients (EW_MX_XUPD)
+    const int *keep_if = nullptr; // EW_AXPBY_Z: leave z unchanged while *keep_if != 0
 };
 
 struct LaunchCfg {

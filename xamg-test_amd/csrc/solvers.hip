@@ -393,17 +393,15 @@ class Chebyshev : public Solver<T> {
             a.y = xzero ? nullptr : x;
             ew<T>(EW_CHEB0_LAST, nv, a, H);
         } else {
+            // The first Horner step gathers the right-hand side itself and forms
+            // t1 = c_{k-1} (b d) scaled by d on the fly, so no separate start pass.
             T *t1 = this->vec(1), *t2 = this->vec(2);
-            EwArgs<T> a;
-            a.n = A->n;
-            a.z = t1;
-            a.x = in;
-            a.d = d;
-            a.c0 = c[order - 1];
-            ew<T>(EW_CHEB0, nv, a, H);
             for (int q = order - 2; q >= 0; --q) {
                 SpmvArgs<T> s{};
-                s.x = t1;
+                const bool first = (q == order - 2);
+                s.x = first ? in : t1;
+                s.x_is_rhs = first ? 1 : 0;
+                s.xcoef = c[order - 1];
                 s.b = in;
                 s.d = d;
                 s.dcol = d;
@@ -699,6 +697,184 @@ class IterRefine : public Solver<double> {
 };
 
 // ---------------------------------------------------------------------------
+// Continued refinement: IterativeRefinement (ir.inl) with refinement = continued.
+// The inner (P)BiCGStab recurrences, the preconditioner and the matrix values run in
+// FP32; the solution is accumulated in FP64 and, whenever the recurrence residual has
+// dropped by `replace_step` since the last FP64 check (or claims convergence), the
+// FP64 residual b - A x replaces the FP32 one (reliable updates) instead of restarting
+// the inner solver as the reference's IR loop does.  Convergence is decided on the FP64
+// residual only.  Reported iterations: FP64 checks (outer) and FP32 iterations (inner).
+// ---------------------------------------------------------------------------
+class ContinuedRefine : public Solver<double> {
+  public:
+    bool pre = false;
+    std::unique_ptr<Solver<float>> precond;
+    std::vector<DBuf> b32;
+    DBuf st32;
+    KrylovScalars<float> *k32 = nullptr;
+    double delta2 = 1e-4;
+    int max_inner = 1000;
+
+    float *v32(int k) { return b32[k].as<float>(); }
+
+    void setup() override {
+        Solver<double>::setup();
+        XAMG_REQUIRE(gp->has("solver"), "IterativeRefinement needs a solver block");
+        const ParamList &sp = gp->get("solver");
+        const std::string m = sp.get_string("method");
+        XAMG_REQUIRE(m == "BiCGStab" || m == "PBiCGStab" || m == "MergedBiCGStab" ||
+                         m == "MergedPBiCGStab",
+                     "continued refinement runs a BiCGStab-family inner solver");
+        pre = (m == "PBiCGStab" || m == "MergedPBiCGStab");
+        max_inner = sp.has("max_iters") ? (int)sp.get_int("max_iters") : 1000;
+        double step = prm.has("replace_step") ? prm.get_float("replace_step") : 1e-2;
+        delta2 = step * step;
+        if (!conv_check)
+            conv_check = 1; // convergence is always decided on the FP64 residual
+        hist_cap = std::max(2, max_inner + 2);
+        histbuf.alloc(sizeof(double) * hist_cap * nv);
+        b32.clear();
+        for (int i = 0; i < (pre ? 8 : 6); i++)
+            b32.emplace_back(sizeof(float) * std::max<int64_t>(A->n, 1) * nv);
+        for (int i = 0; i < (pre ? 8 : 6); i++)
+            fill<float>(v32(i), A->n, nv, 0.0f);
+        st32.alloc(sizeof(KrylovScalars<float>) + 256);
+        k32 = st32.as<KrylovScalars<float>>();
+        A->values(Prec::F32);
+        if (pre) {
+            XAMG_REQUIRE(gp->has("preconditioner"), "PBiCGStab needs a preconditioner block");
+            const ParamList &pp = gp->get("preconditioner");
+            precond = make_solver<float>(pp.get_string("method"), A, pp, "preconditioner", nv, gp, 0);
+        }
+    }
+    void renew(const GlobalParams &g) override {
+        Solver<double>::renew(g);
+        if (precond)
+            precond->renew(g);
+    }
+    FinArgs<float> fin32(int f) {
+        FinArgs<float> a;
+        a.ks = k32;
+        a.cs = cs;
+        a.hist = histbuf.as<double>();
+        a.fin = f;
+        a.conv_check = 1;
+        a.abs_tol = abs_tol;
+        a.rel_tol = rel_tol;
+        a.delta2 = delta2;
+        return a;
+    }
+    FinArgs<double> fin64(int f) {
+        FinArgs<double> a;
+        a.cs = cs;
+        a.fin = f;
+        a.conv_check = 1;
+        a.abs_tol = abs_tol;
+        a.rel_tol = rel_tol;
+        a.aux = k32;
+        a.hist = (f == FIN_MX_INIT) ? histbuf.as<double>() : nullptr; // initial row
+        return a;
+    }
+    void apply(double *x, bool xzero, const double *b, const int *) override {
+        const int nv = this->nv;
+        state_init<float>(k32, cs, nv, hist_cap);
+        if (xzero)
+            fill<double>(x, A->n, nv, 0.0);
+        float *r0 = v32(0), *r = v32(1), *p = v32(2), *v = v32(3), *s = v32(4), *t = v32(5);
+        float *p_ = pre ? v32(6) : p, *s_ = pre ? v32(7) : s;
+        const int *H = &cs->halt;
+        { // FP64 r = b - A x stored as FP32 r0, r, p; FP64 norm and initial check
+            SpmvArgs<double> a{};
+            a.x = x;
+            a.b = b;
+            a.yo = r0;
+            a.yo2 = r;
+            a.yo3 = p;
+            spmv<double>(*A, Prec::F64, EPI_RESID_MX, 1, nv, a, red_args(nullptr), fin64(FIN_MX_INIT));
+        }
+        auto iteration = [&]() {
+            if (pre)
+                precond->apply(p_, true, p, H);
+            {
+                SpmvArgs<float> a{};
+                a.x = p_;
+                a.y = v;
+                a.w = r0;
+                spmv<float>(*A, Prec::F32, EPI_AXY, 1, nv, a, red_args(H), fin32(FIN_ALPHA));
+            }
+            {
+                EwArgs<float> a;
+                a.n = A->n;
+                a.z = s;
+                a.x = r;
+                a.y = v;
+                a.b = k32->nalpha;
+                ew<float>(EW_AXPBY_Z, nv, a, H);
+            }
+            if (pre)
+                precond->apply(s_, true, s, H);
+            {
+                SpmvArgs<float> a{};
+                a.x = s_;
+                a.y = t;
+                a.w = s;
+                spmv<float>(*A, Prec::F32, EPI_AXY, 3, nv, a, red_args(H), fin32(FIN_MX_OMEGA));
+            }
+            { // x(FP64) += alpha_c p_ + omega_c s_
+                EwArgs<double> a;
+                a.n = A->n;
+                a.z = x;
+                a.xo = p_;
+                a.yo = s_;
+                a.ao = k32->ac;
+                a.bo = k32->oc;
+                ew<double>(EW_MX_XUPD, nv, a, H);
+            }
+            { // FP64 residual replacement when requested
+                SpmvArgs<double> a{};
+                a.x = x;
+                a.b = b;
+                a.yo = r;
+                RedArgs ra = red_args(H);
+                ra.gate = &cs->replace;
+                spmv<double>(*A, Prec::F64, EPI_RESID_MX, 1, nv, a, ra, fin64(FIN_MX_REPL));
+            }
+            { // r = s - omega t (kept when just replaced), rho = (r, r0), beta
+                EwArgs<float> a;
+                a.n = A->n;
+                a.z = r;
+                a.x = s;
+                a.y = t;
+                a.b = k32->nomega;
+                a.w = r0;
+                a.keep_if = &cs->replaced;
+                RedArgs ra = red_args(H);
+                ra.set_halt_if_done = 1;
+                ra.cs = cs;
+                FinArgs<float> fa = fin32(FIN_BETA);
+                fa.clear_on_fin = &cs->replaced;
+                ew<float>(EW_AXPBY_Z, 1, nv, a, ra, fa);
+            }
+            {
+                EwArgs<float> a;
+                a.n = A->n;
+                a.z = p;
+                a.x = r;
+                a.y = v;
+                a.b = k32->nbo;
+                a.c = k32->beta;
+                ew<float>(EW_AXPBYPCZ, nv, a, H);
+            }
+        };
+        run_loop(max_inner, cs, iteration);
+    }
+    void stats(SolveStats &st, bool with_hist) override {
+        Solver<double>::stats(st, with_hist);
+        std::swap(st.iters, st.sub_iters); // outer = FP64 checks, inner = FP32 iterations
+    }
+};
+
+// ---------------------------------------------------------------------------
 // MultiGrid (mg.inl:117-331) over a host-built hierarchy
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -853,9 +1029,16 @@ std::unique_ptr<Solver<T>> make_solver(const std::string &method, DevMatrix *A,
         m->hier = hier;
         s.reset(m);
     } else if (method == "IterativeRefinement") {
-        if constexpr (std::is_same<T, double>::value)
-            s.reset(new IterRefine());
-        else
+        if constexpr (std::is_same<T, double>::value) {
+            const bool continued = prm.has("refinement") &&
+                                   prm.get_string("refinement") == "continued" &&
+                                   prm.has("subsolver_reduced_precision") &&
+                                   prm.get_bool("subsolver_reduced_precision");
+            if (continued)
+                s.reset(new ContinuedRefine());
+            else
+                s.reset(new IterRefine());
+        } else
             throw std::runtime_error("xamg: IterativeRefinement must be the FP64 meta solver");
     } else {
         throw std::runtime_error("xamg: the selected method '" + method + "' is not implemented");
