@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV (FP32 and FP64, nv) and copy-kernel timings; used for "
+                         "the hardware-counter passes of the SpMV roofline")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -124,6 +127,16 @@ def main():
     c = np.cos(np.pi / (n1 + 1))
     eigs = (1.0 - c, 1.0 + c)  # closed form for the 7-point Dirichlet Laplacian
     A.set_eigs(*eigs)
+    if args.spmv_only:
+        out = {}
+        for prec, sv in ((1, 4), (0, 8)):
+            ms = A.bench_spmv(nv, prec=prec, iters=200)
+            out[f"prec{prec}_ms"] = ms
+            out[f"prec{prec}_gbs"] = spmv_bytes(n, nnz, nv, sv, sv) / (ms * 1e-3) / 1e9
+        out["copy_1GiB_gbs"] = X.bench_copy(1 << 30, iters=20)
+        print(json.dumps(out), flush=True)
+        return
+
     xv, bv = X.Vector(n, nv), X.Vector(n, nv)
     bv.upload(b)
 
@@ -177,6 +190,13 @@ def main():
     gbs64 = b64 / (ms64 * 1e-3) / 1e9
     copy_gbs = X.bench_copy(2 << 30, iters=20)
 
+    # HBM bytes per launch of the same kernel and workload from the committed rocprofv3
+    # counter passes (FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated)
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"spmv_7pt{n1}_nv{nv}.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof))["fp32"]["traffic_bytes"]
+
     mean_mx = float(np.mean(tmx))
     mean_64 = float(np.mean(t64))
     line = {
@@ -209,7 +229,7 @@ def main():
                  "copy_gbs": copy_gbs, "fp32_frac_of_copy": gbs32 / copy_gbs,
                  "fp64_frac_of_copy": gbs64 / copy_gbs},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_spmv<float,float,8,EPI_AXY,0>",
                      "algorithmic_bytes": b32},
         "device": dev,

@@ -695,7 +695,13 @@ __global__ void __launch_bounds__(64)
 template <typename V, typename T, int NV, int VPT>
 static void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
                      hipStream_t s) {
-    const LaunchCfg &c = launch_cfg();
+    LaunchCfg c = launch_cfg();
+    if (a.cfg_grid > 0) {
+        c.grid_spmv = a.cfg_grid;
+        c.grid_red = a.cfg_grid;
+    }
+    if (a.cfg_remap >= 0)
+        c.xcd_remap = a.cfg_remap;
     int G = nd > 0 ? c.grid_red : c.grid_spmv;
     if (nd > 0 && G > r.ws.max_blocks)
         G = r.ws.max_blocks;
@@ -734,7 +740,7 @@ int &spmv_vpt_choice(int nv, int es) {
 template <typename V, typename T, int NV>
 static void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
                     hipStream_t s) {
-    int vpt = spmv_vpt_choice(NV, (int)sizeof(T));
+    int vpt = a.cfg_vpt > 0 ? a.cfg_vpt : spmv_vpt_choice(NV, (int)sizeof(T));
     if constexpr (NV == 1) {
         spmv_nvv<V, T, 1, 1>(epi, nd, a, r, f, s);
     } else if constexpr (NV == 2) {

@@ -103,6 +103,8 @@ struct SpmvArgs {
     T *y2 = nullptr, *y3 = nullptr; // extra copies of y (initial residual: r, p)
     // EPI_RESID_MX: FP32 copies of the FP64 residual (y is not written)
     float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr;
+    // launch configuration chosen by the autotuner (0 / -1: global defaults)
+    int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1;
 };
 
 template <typename T>

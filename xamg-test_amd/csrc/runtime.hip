@@ -277,6 +277,12 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
     a.row = A.row.as<uint32_t>();
     a.col = A.col.as<uint32_t>();
     a.n = A.n;
+    if (a.cfg_vpt == 0 && a.cfg_grid == 0) {
+        const DevMatrix::Tune &t = A.tuned(vp, (int)sizeof(T), nv);
+        a.cfg_vpt = t.vpt;
+        a.cfg_grid = t.grid;
+        a.cfg_remap = t.remap;
+    }
     if constexpr (std::is_same<T, double>::value) {
         XAMG_REQUIRE(vp == Prec::F64, "FP64 vectors need FP64 matrix values");
         a.val = A.values(Prec::F64);

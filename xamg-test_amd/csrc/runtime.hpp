@@ -92,7 +92,20 @@ struct DevMatrix {
     // Extreme eigenvalues of D^{-1/2} A D^{-1/2} by Lanczos on the host (used when the
     // caller supplies no bounds; hypre's setup supplies them in the reference).
     void estimate_eigs(int steps = 60);
+
+    // SpMV launch configuration per (value precision, vector element size, nv), chosen
+    // by timing the candidates on this matrix (autotune.cpp).
+    struct Tune {
+        int vpt = 0, grid = 0, remap = -1;
+        bool valid = false;
+    };
+    Tune tune[3][2][kMaxNV + 1];
+    const Tune &tuned(Prec vp, int es, int nv);
 };
+
+// Autotuner switches: XAMG_AUTOTUNE=0 disables it; XAMG_TUNE_FILE names a text file
+// that caches results across processes.
+bool autotune_enabled();
 
 // y = A x dispatch on the value precision of the matrix copy in use.
 template <typename T>
