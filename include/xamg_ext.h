@@ -77,6 +77,42 @@ int xamgx_set_spmv_vpt(int nv, int es, int vpt);
 int xamgx_matrix_set_eigs(XAMG_matrix_id mid, double lo, double hi);
 int xamgx_matrix_get_eigs(XAMG_matrix_id mid, double *lo, double *hi);
 
+/* ---- multigrid hierarchies ----
+ * The reference builds its MultiGrid levels with hypre BoomerAMG and imports A_l,
+ * P_l, R_l = P_l^T and spectrum estimates (hypre_base.inl:340-406).  Here a host
+ * classical AMG setup (Ruge-Stueben coarsening, classical or direct interpolation,
+ * Galerkin products) driven by the same MultiGrid keys builds them, or a caller
+ * imports its own levels.  Level l >= 1 holds A_l; P_l (l < levels-1) interpolates
+ * level l+1 to level l; level 0 is the caller's matrix.  Indices are int like
+ * XAMG_csr_matrix_fill.  Setup runs on the host and needs no GPU. */
+typedef size_t XAMG_hierarchy_id;
+/* Setup from a constructed matrix and the MultiGrid list of a parameter dictionary
+ * (mg_strength_threshold, mg_max_row_sum, mg_coarse_matrix_size, mg_max_levels,
+ * mg_interpolation_type 0 classical / 3 direct, mg_trunc_factor, mg_Pmax_elements). */
+int xamgx_hierarchy_build(XAMG_matrix_id mid, XAMG_param_id pid, XAMG_hierarchy_id *hid);
+/* Same setup from host CSR arrays (nrows x nrows). */
+int xamgx_hierarchy_build_csr(XAMG_param_id pid, uint64_t nrows, const int *row, const int *col,
+                              const double *val, XAMG_hierarchy_id *hid);
+/* Empty hierarchy, then one call per coarse level: P (nfine x ncoarse) and A_coarse. */
+int xamgx_hierarchy_create(XAMG_hierarchy_id *hid);
+int xamgx_hierarchy_add_level(XAMG_hierarchy_id hid, uint64_t nfine, uint64_t ncoarse,
+                              const int *prow, const int *pcol, const double *pval,
+                              const int *arow, const int *acol, const double *aval);
+int xamgx_hierarchy_num_levels(XAMG_hierarchy_id hid, int *nlevels);
+/* sizes = {rows of level l, nnz(A_l) (0 for l = 0), columns of P_l, nnz(P_l)} */
+int xamgx_hierarchy_level_size(XAMG_hierarchy_id hid, int level, uint64_t sizes[4]);
+/* Copies A_l (l >= 1) and P_l (l < levels-1) out; a NULL pointer skips that array. */
+int xamgx_hierarchy_level_get(XAMG_hierarchy_id hid, int level, int *arow, int *acol, double *aval,
+                              int *prow, int *pcol, double *pval);
+/* Spectrum bounds of D^{-1/2} A_l D^{-1/2} (l >= 1) for Chebyshev smoothers; get
+ * estimates them by Lanczos when unset (the estimate a solver would make). */
+int xamgx_hierarchy_set_eigs(XAMG_hierarchy_id hid, int level, double lo, double hi);
+int xamgx_hierarchy_get_eigs(XAMG_hierarchy_id hid, int level, double *lo, double *hi);
+/* MultiGrid solvers created on the matrix afterwards use this hierarchy (uploaded to
+ * HBM now) instead of running the setup. */
+int xamgx_matrix_set_hierarchy(XAMG_matrix_id mid, XAMG_hierarchy_id hid);
+int xamgx_hierarchy_destroy(XAMG_hierarchy_id hid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,3 +77,30 @@ def bf16_round(a):
     lsb = (f >> 16) & 1
     f = ((f + 0x7FFF + lsb) >> 16) << 16
     return f.astype(np.uint32).view(np.float32)
+
+
+def hierarchy_levels(H):
+    """Levels of a pyxamg Hierarchy as scipy matrices: ([A_1..], [P_0..])."""
+    import scipy.sparse as sp
+    As, Ps = [], []
+    for l in range(H.levels()):
+        L = H.level(l)
+        if "A" in L:
+            r, c, v = L["A"]
+            As.append(sp.csr_matrix((v, c, r), shape=(L["n"], L["n"])))
+        if "P" in L:
+            r, c, v, nc = L["P"]
+            Ps.append(sp.csr_matrix((v, c, r), shape=(L["n"], nc)))
+    return As, Ps
+
+
+def oracle_hierarchy(H, A0, eigs0, reduced=False):
+    """The same levels in the oracle's layout, with the library's spectrum estimates."""
+    import oracle as O
+    As, Ps = hierarchy_levels(H)
+    lo, hi = [eigs0[0]], [eigs0[1]]
+    for l in range(1, H.levels()):
+        e = H.eigs(l)
+        lo.append(e[0])
+        hi.append(e[1])
+    return O.Hierarchy([A0] + As, Ps, lo, hi, reduced=reduced)

@@ -91,8 +91,9 @@ float time_cfg(DevMatrix &A, Prec vp, int nv, const T *x, T *y, int vpt, int gri
 
 template <typename T>
 DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv) {
-    DBuf x(sizeof(T) * std::max<int64_t>(A.n, 1) * nv), y(sizeof(T) * std::max<int64_t>(A.n, 1) * nv);
-    fill<T>(x.as<T>(), A.n, nv, (T)1);
+    // x has one row per column of A (the transfer operators of a hierarchy are rectangular)
+    DBuf x(sizeof(T) * std::max<int64_t>(A.ncols, 1) * nv), y(sizeof(T) * std::max<int64_t>(A.n, 1) * nv);
+    fill<T>(x.as<T>(), A.ncols, nv, (T)1);
     DevMatrix::Tune best;
     float best_ms = 1e30f;
     for (int vpt : vpt_options(nv))
@@ -121,7 +122,7 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv) {
     if (!autotune_enabled() || n < 65536)
         return t;
     std::ostringstream k;
-    k << "n" << n << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv" << nv;
+    k << "n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv" << nv;
     const std::string key = k.str();
     auto &c = cache();
     auto it = c.find(key);

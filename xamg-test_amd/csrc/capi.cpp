@@ -76,6 +76,51 @@ std::vector<Vec *> g_vecs;
 std::vector<HostCsr *> g_csr;
 std::vector<Mat *> g_mats;
 std::vector<Sol *> g_sols;
+std::vector<amg::Hierarchy *> g_hiers;
+
+// MultiGrid keys of a parameter dictionary: the first list whose method is MultiGrid,
+// with the reference defaults applied (hypre_base.h:124-215 reads the same keys).
+ParamList mg_list(const GlobalParams &p) {
+    GlobalParams g = p;
+    g.set_defaults();
+    for (auto &r : g.roles)
+        if (r.second.has("method") && r.second.get_string("method") == "MultiGrid")
+            return r.second;
+    ParamList d;
+    set_family_defaults(d, "MultiGrid", "preconditioner");
+    return d;
+}
+
+void copy_csr_out(const amg::Csr &c, int *row, int *col, double *val) {
+    if (row)
+        for (int64_t i = 0; i <= c.n; i++)
+            row[i] = (int)c.row[i];
+    if (col)
+        for (int64_t j = 0; j < c.nnz(); j++)
+            col[j] = (int)c.col[j];
+    if (val)
+        std::copy(c.val.begin(), c.val.end(), val);
+}
+
+amg::Csr csr_in(int64_t n, int64_t ncols, const int *row, const int *col, const double *val) {
+    amg::Csr c;
+    c.n = n;
+    c.ncols = ncols;
+    c.row.resize(n + 1);
+    XAMG_REQUIRE(row[0] == 0, "row pointers must start at 0");
+    for (int64_t i = 0; i <= n; i++) {
+        XAMG_REQUIRE(i == 0 || row[i] >= row[i - 1], "row pointers must be non-decreasing");
+        c.row[i] = (uint32_t)row[i];
+    }
+    const int64_t nnz = row[n];
+    c.col.resize(nnz);
+    c.val.assign(val, val + nnz);
+    for (int64_t j = 0; j < nnz; j++) {
+        XAMG_REQUIRE(col[j] >= 0 && col[j] < ncols, "column index out of range");
+        c.col[j] = (uint32_t)col[j];
+    }
+    return c;
+}
 
 template <typename T>
 T *slot(std::vector<T *> &v, size_t id, const char *what) {
@@ -869,3 +914,125 @@ extern "C" int xamgx_bench_copy(uint64_t bytes, int iters, double *gbs) {
         (void)hipEventDestroy(e1);
     });
 }
+
+// ---- AMG hierarchies ----
+// Level l >= 1 carries A_l; P_l (l < levels-1) maps level l+1 to level l.  The fine
+// level's rows are recorded in the first interpolation operator.
+extern "C" {
+
+int xamgx_hierarchy_build(XAMG_matrix_id mid, XAMG_param_id pid, XAMG_hierarchy_id *hid) {
+    return guard([&] {
+        DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
+        XAMG_REQUIRE(A != nullptr, "matrix not constructed");
+        GlobalParams *p = slot(g_params, pid, "param");
+        amg::Csr A0;
+        A0.n = A->n;
+        A0.ncols = A->ncols;
+        A0.row = A->hrow;
+        A0.col = A->hcol;
+        A0.val = A->hval;
+        g_hiers.push_back(new amg::Hierarchy(amg::setup(A0, amg::options_from(mg_list(*p)))));
+        *hid = g_hiers.size() - 1;
+    });
+}
+int xamgx_hierarchy_build_csr(XAMG_param_id pid, uint64_t nrows, const int *row, const int *col,
+                              const double *val, XAMG_hierarchy_id *hid) {
+    return guard([&] {
+        GlobalParams *p = slot(g_params, pid, "param");
+        amg::Csr A0 = csr_in((int64_t)nrows, (int64_t)nrows, row, col, val);
+        g_hiers.push_back(new amg::Hierarchy(amg::setup(A0, amg::options_from(mg_list(*p)))));
+        *hid = g_hiers.size() - 1;
+    });
+}
+int xamgx_hierarchy_create(XAMG_hierarchy_id *hid) {
+    return guard([&] {
+        auto *h = new amg::Hierarchy();
+        h->A.emplace_back();
+        h->min_eig.push_back(std::numeric_limits<double>::quiet_NaN());
+        h->max_eig.push_back(std::numeric_limits<double>::quiet_NaN());
+        g_hiers.push_back(h);
+        *hid = g_hiers.size() - 1;
+    });
+}
+int xamgx_hierarchy_add_level(XAMG_hierarchy_id hid, uint64_t nfine, uint64_t ncoarse,
+                              const int *prow, const int *pcol, const double *pval,
+                              const int *arow, const int *acol, const double *aval) {
+    return guard([&] {
+        amg::Hierarchy *h = slot(g_hiers, hid, "hierarchy");
+        const int l = h->levels() - 1;
+        XAMG_REQUIRE(l == 0 || (int64_t)nfine == h->A[l].n,
+                     "fine rows differ from the previous level's coarse rows");
+        h->P.push_back(csr_in((int64_t)nfine, (int64_t)ncoarse, prow, pcol, pval));
+        h->A.push_back(csr_in((int64_t)ncoarse, (int64_t)ncoarse, arow, acol, aval));
+        h->min_eig.push_back(std::numeric_limits<double>::quiet_NaN());
+        h->max_eig.push_back(std::numeric_limits<double>::quiet_NaN());
+    });
+}
+int xamgx_hierarchy_num_levels(XAMG_hierarchy_id hid, int *nlevels) {
+    return guard([&] { *nlevels = slot(g_hiers, hid, "hierarchy")->levels(); });
+}
+int xamgx_hierarchy_level_size(XAMG_hierarchy_id hid, int level, uint64_t sizes[4]) {
+    return guard([&] {
+        amg::Hierarchy *h = slot(g_hiers, hid, "hierarchy");
+        XAMG_REQUIRE(level >= 0 && level < h->levels(), "level out of range");
+        const bool hasP = level + 1 < h->levels();
+        sizes[0] = hasP ? (uint64_t)h->P[level].n : (uint64_t)h->A[level].n;
+        sizes[1] = level > 0 ? (uint64_t)h->A[level].nnz() : 0;
+        sizes[2] = hasP ? (uint64_t)h->P[level].ncols : 0;
+        sizes[3] = hasP ? (uint64_t)h->P[level].nnz() : 0;
+    });
+}
+int xamgx_hierarchy_level_get(XAMG_hierarchy_id hid, int level, int *arow, int *acol, double *aval,
+                              int *prow, int *pcol, double *pval) {
+    return guard([&] {
+        amg::Hierarchy *h = slot(g_hiers, hid, "hierarchy");
+        XAMG_REQUIRE(level >= 0 && level < h->levels(), "level out of range");
+        if (level > 0)
+            copy_csr_out(h->A[level], arow, acol, aval);
+        if (level + 1 < h->levels())
+            copy_csr_out(h->P[level], prow, pcol, pval);
+    });
+}
+int xamgx_hierarchy_set_eigs(XAMG_hierarchy_id hid, int level, double lo, double hi) {
+    return guard([&] {
+        amg::Hierarchy *h = slot(g_hiers, hid, "hierarchy");
+        XAMG_REQUIRE(level >= 1 && level < h->levels(),
+                     "level out of range (level 0 bounds belong to the matrix)");
+        h->min_eig[level] = lo;
+        h->max_eig[level] = hi;
+    });
+}
+int xamgx_hierarchy_get_eigs(XAMG_hierarchy_id hid, int level, double *lo, double *hi) {
+    return guard([&] {
+        amg::Hierarchy *h = slot(g_hiers, hid, "hierarchy");
+        XAMG_REQUIRE(level >= 1 && level < h->levels(),
+                     "level out of range (level 0 bounds belong to the matrix)");
+        if (std::isnan(h->min_eig[level]) || std::isnan(h->max_eig[level])) {
+            const amg::Csr &A = h->A[level];
+            amg::lanczos_bounds(A.n, A.row.data(), A.col.data(), A.val.data(), 60,
+                                &h->min_eig[level], &h->max_eig[level]);
+        }
+        *lo = h->min_eig[level];
+        *hi = h->max_eig[level];
+    });
+}
+int xamgx_matrix_set_hierarchy(XAMG_matrix_id mid, XAMG_hierarchy_id hid) {
+    return guard([&] {
+        DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
+        XAMG_REQUIRE(A != nullptr, "matrix not constructed");
+        amg::Hierarchy *h = slot(g_hiers, hid, "hierarchy");
+        XAMG_REQUIRE(h->levels() >= 2 && h->P[0].n == A->n,
+                     "hierarchy does not match the matrix (fine rows differ)");
+        A->hier = upload_hierarchy(A, *h);
+    });
+}
+int xamgx_hierarchy_destroy(XAMG_hierarchy_id hid) {
+    return guard([&] {
+        if (hid < g_hiers.size()) {
+            delete g_hiers[hid];
+            g_hiers[hid] = nullptr;
+        }
+    });
+}
+
+} // extern "C"

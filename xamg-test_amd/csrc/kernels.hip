@@ -725,8 +725,10 @@ static void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, co
     XAMG_SPMV_CASE(EPI_JACOBI, 0)
     XAMG_SPMV_CASE(EPI_CHEB, 0)
     XAMG_SPMV_CASE(EPI_CHEB_LAST, 0)
-    if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value)
+    if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value) {
+        XAMG_SPMV_CASE(EPI_RESID_MX, 0)
         XAMG_SPMV_CASE(EPI_RESID_MX, 1)
+    }
 #undef XAMG_SPMV_CASE
     throw std::runtime_error("xamg: unsupported spmv epilogue/dot combination");
 }

@@ -1,5 +1,6 @@
 // Host runtime: stream, reduction workspace, device CSR copies (see runtime.hpp).
 #include "runtime.hpp"
+#include "amg.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -187,87 +188,9 @@ const float *DevMatrix::dense_inverse<float>() {
 // Lanczos on S = D^{-1/2} A D^{-1/2} (symmetric for symmetric A); the extreme Ritz
 // values of the tridiagonal matrix bound the spectrum used by Chebyshev.
 void DevMatrix::estimate_eigs(int steps) {
-    const int64_t N = n;
-    if (N == 0)
+    if (n == 0)
         return;
-    std::vector<double> d(N, 1.0);
-    for (int64_t i = 0; i < N; i++)
-        for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
-            if (hcol[j] == (uint32_t)i)
-                d[i] = std::sqrt(1.0 / hval[j]);
-    steps = (int)std::min<int64_t>(steps, N);
-    std::vector<double> v(N), vprev(N, 0.0), w(N);
-    // deterministic start vector
-    double nrm = 0;
-    for (int64_t i = 0; i < N; i++) {
-        v[i] = 1.0 + 0.5 * std::sin(0.7 * (double)i);
-        nrm += v[i] * v[i];
-    }
-    nrm = std::sqrt(nrm);
-    for (auto &x : v)
-        x /= nrm;
-    std::vector<double> alpha, beta;
-    double b_prev = 0;
-    for (int k = 0; k < steps; k++) {
-        for (int64_t i = 0; i < N; i++) {
-            double s = 0;
-            for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
-                s += hval[j] * d[hcol[j]] * v[hcol[j]];
-            w[i] = d[i] * s;
-        }
-        double a = 0;
-        for (int64_t i = 0; i < N; i++)
-            a += w[i] * v[i];
-        for (int64_t i = 0; i < N; i++)
-            w[i] -= a * v[i] + b_prev * vprev[i];
-        double b = 0;
-        for (int64_t i = 0; i < N; i++)
-            b += w[i] * w[i];
-        b = std::sqrt(b);
-        alpha.push_back(a);
-        if (b < 1e-14 * std::fabs(a) || k == steps - 1)
-            break;
-        beta.push_back(b);
-        for (int64_t i = 0; i < N; i++) {
-            vprev[i] = v[i];
-            v[i] = w[i] / b;
-        }
-        b_prev = b;
-    }
-    // eigenvalues of the tridiagonal matrix by bisection (Sturm sequences)
-    const int m = (int)alpha.size();
-    auto count_below = [&](double x) {
-        int c = 0;
-        double q = alpha[0] - x;
-        if (q < 0)
-            c++;
-        for (int k = 1; k < m; k++) {
-            double bb = beta[k - 1] * beta[k - 1];
-            q = alpha[k] - x - bb / (q != 0 ? q : 1e-300);
-            if (q < 0)
-                c++;
-        }
-        return c;
-    };
-    double lo = 1e300, hi = -1e300;
-    for (int k = 0; k < m; k++) {
-        double r = (k > 0 ? std::fabs(beta[k - 1]) : 0) + (k < m - 1 ? std::fabs(beta[k]) : 0);
-        lo = std::min(lo, alpha[k] - r);
-        hi = std::max(hi, alpha[k] + r);
-    }
-    auto kth = [&](int k) {
-        double a = lo, b = hi;
-        for (int it = 0; it < 200; it++) {
-            double mid = 0.5 * (a + b);
-            if (count_below(mid) > k)
-                b = mid;
-            else
-                a = mid;
-        }
-        return 0.5 * (a + b);
-    };
-    min_eig = kth(0);
-    max_eig = kth(m - 1);
+    amg::lanczos_bounds(n, hrow.data(), hcol.data(), hval.data(), steps, &min_eig, &max_eig);
 }
 
 template <typename T>

@@ -13,6 +13,8 @@
 
 namespace xamg {
 
+struct MgHierarchy;
+
 // Owning HBM allocation (mem.h:87-126 alloc_buffer, rebuilt on hipMalloc: 256-byte
 // aligned device memory instead of malloc / POSIX shm segments).
 struct DBuf {
@@ -80,6 +82,9 @@ struct DevMatrix {
     DBuf dense_inv64, dense_inv32; // Direct: column-major inverse (inverse.inl:108-168)
     double min_eig = std::numeric_limits<double>::quiet_NaN();
     double max_eig = std::numeric_limits<double>::quiet_NaN();
+    // AMG hierarchy attached by the caller; MultiGrid solvers on this matrix use it
+    // instead of running the setup.
+    std::shared_ptr<MgHierarchy> hier;
 
     void upload();
     const void *values(Prec p);

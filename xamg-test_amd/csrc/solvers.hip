@@ -877,13 +877,120 @@ class ContinuedRefine : public Solver<double> {
 // ---------------------------------------------------------------------------
 // MultiGrid (mg.inl:117-331) over a host-built hierarchy
 // ---------------------------------------------------------------------------
+template <typename U>
+constexpr Prec prec_of() {
+    return std::is_same<U, double>::value ? Prec::F64 : Prec::F32;
+}
+
+// Smoothers, coarse solver and work vectors of the levels one precision covers
+// (mg.inl:131-183: three buffers per level, the level's own precision).
+template <typename U>
+struct MgLevels {
+    std::vector<std::unique_ptr<Solver<U>>> pre, post; // by level; null outside the range
+    std::unique_ptr<Solver<U>> coarse;
+    std::vector<DBuf> temp, bH, xH;
+
+    void build(MgHierarchy &H, int lb, int le, bool with_coarse, int nv, const GlobalParams *g) {
+        const int nl = (int)H.levels.size();
+        pre.resize(nl);
+        post.resize(nl);
+        temp.resize(nl);
+        bH.resize(nl);
+        xH.resize(nl);
+        for (int l = lb; l < le && l < nl - 1; l++) {
+            ParamList pp = g->get_level("pre_smoother", l), qp = g->get_level("post_smoother", l);
+            DevMatrix *A = H.levels[l].A;
+            pre[l] = make_solver<U>(pp.get_string("method"), A, pp, "pre_smoother", nv, g, l);
+            post[l] = make_solver<U>(qp.get_string("method"), A, qp, "post_smoother", nv, g, l);
+        }
+        if (with_coarse) {
+            const ParamList &cp = g->get("coarse_grid_solver");
+            coarse = make_solver<U>(cp.get_string("method"), H.levels[nl - 1].A, cp,
+                                    "coarse_grid_solver", nv, g, nl - 1);
+        }
+        for (int l = lb; l < le; l++) {
+            const size_t sz = sizeof(U) * std::max<int64_t>(H.levels[l].A->n, 1) * nv;
+            temp[l].alloc(sz);
+            if (l > 0) {
+                bH[l].alloc(sz);
+                xH[l].alloc(sz);
+            }
+        }
+    }
+    void renew(const GlobalParams &g) {
+        for (auto &s : pre)
+            if (s)
+                s->renew(g);
+        for (auto &s : post)
+            if (s)
+                s->renew(g);
+        if (coarse)
+            coarse->renew(g);
+    }
+};
+
+// MG_cycle<U,U> (mg.inl:186-280) on levels lev.. of one precision.
+template <typename U>
+static void mg_cycle_uniform(MgHierarchy &H, MgLevels<U> &L, int nv, U *x, bool xzero, const U *b,
+                             int lev, int cyc, const int *halt) {
+    constexpr Prec P = prec_of<U>();
+    DevMatrix *A = H.levels[lev].A;
+    U *tmp = L.temp[lev].template as<U>();
+    U *bh = L.bH[lev + 1].template as<U>();
+    U *xh = L.xH[lev + 1].template as<U>();
+    if (cyc != 1) {
+        L.pre[lev]->apply(x, xzero, b, halt);
+        xzero = false;
+    }
+    if (xzero) {
+        // temp = 1*b (A*0 is the zero vector)
+        EwArgs<U> a;
+        a.n = A->n;
+        a.z = tmp;
+        a.x = b;
+        a.y2 = tmp;
+        ew<U>(EW_COPY3, nv, a, halt);
+    } else {
+        SpmvArgs<U> a{};
+        a.x = x;
+        a.y = tmp;
+        a.b = b;
+        spmv<U>(*A, P, EPI_RESID, 0, nv, a, red_args(halt), FinArgs<U>{});
+    }
+    {
+        SpmvArgs<U> a{};
+        a.x = tmp;
+        a.y = bh;
+        spmv<U>(*H.levels[lev].R, P, EPI_AXY, 0, nv, a, red_args(halt), FinArgs<U>{});
+    }
+    if (lev + 1 == (int)H.levels.size() - 1) {
+        L.coarse->apply(xh, true, bh, halt);
+    } else {
+        mg_cycle_uniform<U>(H, L, nv, xh, true, bh, lev + 1, cyc, halt);
+        if (cyc == 2)
+            mg_cycle_uniform<U>(H, L, nv, xh, false, bh, lev + 1, 2, halt);
+        else if (cyc == 1)
+            mg_cycle_uniform<U>(H, L, nv, xh, false, bh, lev + 1, 0, halt);
+    }
+    if (xzero)
+        fill<U>(x, A->n, nv, (U)0, halt);
+    {
+        SpmvArgs<U> a{};
+        a.x = xh;
+        a.y = x;
+        spmv<U>(*H.levels[lev].P, P, EPI_AXPY, 0, nv, a, red_args(halt), FinArgs<U>{});
+    }
+    L.post[lev]->apply(x, false, b, halt);
+}
+
 template <typename T>
 class MultiGrid : public Solver<T> {
   public:
     std::shared_ptr<MgHierarchy> hier;
-    std::vector<std::unique_ptr<Solver<T>>> pre, post;
-    std::unique_ptr<Solver<T>> coarse;
-    std::vector<DBuf> temp, bH, xH;
+    MgLevels<T> lv;       // every level (uniform), or level 0 (reduced precision)
+    MgLevels<float> lv32; // levels >= 1 under mg_reduced_precision
+    DBuf temp32;          // level-0 residual / prolongation in FP32
+    bool mixed = false;
     int cycle = 0;
 
     void setup() override {
@@ -891,101 +998,105 @@ class MultiGrid : public Solver<T> {
         this->alloc_bufs(1);
         if (!hier)
             hier = build_amg(this->A, this->prm);
-        auto &L = hier->levels;
-        const int nl = (int)L.size();
+        const int nl = (int)hier->levels.size();
         XAMG_REQUIRE(nl >= 2, "MultiGrid hierarchy needs at least two levels");
+        XAMG_REQUIRE(hier->levels[0].A == this->A, "MultiGrid hierarchy belongs to another matrix");
         cycle = (int)this->prm.get_int("mg_cycle");
-        const GlobalParams *g = this->gp;
-        for (int l = 0; l < nl - 1; l++) {
-            ParamList pp = g->get_level("pre_smoother", l), qp = g->get_level("post_smoother", l);
-            pre.push_back(make_solver<T>(pp.get_string("method"), L[l].A, pp, "pre_smoother",
-                                         this->nv, g, l));
-            post.push_back(make_solver<T>(qp.get_string("method"), L[l].A, qp, "post_smoother",
-                                          this->nv, g, l));
-        }
-        const ParamList &cp = g->get("coarse_grid_solver");
-        coarse = make_solver<T>(cp.get_string("method"), L[nl - 1].A, cp, "coarse_grid_solver",
-                                this->nv, g, nl - 1);
-        temp.resize(nl);
-        bH.resize(nl);
-        xH.resize(nl);
-        for (int l = 0; l < nl; l++) {
-            size_t sz = sizeof(T) * std::max<int64_t>(L[l].A->n, 1) * this->nv;
-            temp[l].alloc(sz);
-            bH[l].alloc(sz);
-            xH[l].alloc(sz);
+        mixed = std::is_same<T, double>::value && this->prm.has("mg_reduced_precision") &&
+                this->prm.get_bool("mg_reduced_precision");
+        if (!mixed) {
+            lv.build(*hier, 0, nl, true, this->nv, this->gp);
+        } else {
+            // mg.inl:131-151: level 0 keeps the solver precision, levels >= 1 (coarse
+            // solver included) and every transfer operator run in FP32
+            lv.build(*hier, 0, 1, false, this->nv, this->gp);
+            lv32.build(*hier, 1, nl, true, this->nv, this->gp);
+            temp32.alloc(sizeof(float) * std::max<int64_t>(this->A->n, 1) * this->nv);
         }
     }
     void renew(const GlobalParams &g) override {
         Solver<T>::renew(g);
-        for (auto &s : pre)
-            s->renew(g);
-        for (auto &s : post)
-            s->renew(g);
-        coarse->renew(g);
+        lv.renew(g);
+        lv32.renew(g);
     }
-    // mg.inl:186-280 MG_cycle (uniform precision)
-    void mg_cycle(T *x, bool xzero, const T *b, int lev, int cyc, const int *H) {
-        auto &L = hier->levels;
-        const int nv = this->nv;
-        DevMatrix *A = L[lev].A;
-        T *tmp = temp[lev].template as<T>();
-        T *bh = bH[lev + 1].template as<T>();
-        T *xh = xH[lev + 1].template as<T>();
-        if (cyc != 1) {
-            pre[lev]->apply(x, xzero, b, H);
-            xzero = false;
+    // MG_cycle<double,float> at level 0 (mg.inl:212-226, 246-256)
+    void mg_cycle_mixed(T *x, bool xzero, const T *b, int cyc, const int *H) {
+        if constexpr (std::is_same<T, double>::value) {
+            MgHierarchy &Hh = *hier;
+            const int nv = this->nv;
+            DevMatrix *A = Hh.levels[0].A;
+            float *t32 = temp32.as<float>();
+            float *bh = lv32.bH[1].template as<float>();
+            float *xh = lv32.xH[1].template as<float>();
+            if (cyc != 1) {
+                lv.pre[0]->apply(x, xzero, b, H);
+                xzero = false;
+            }
+            if (xzero) {
+                EwArgs<float> c;
+                c.n = A->n;
+                c.z = t32;
+                c.xo = b;
+                ew<float>(EW_CONVERT, nv, c, H);
+            } else {
+                // temp = 1*b + (-1)*A x in FP64, stored as FP32
+                SpmvArgs<double> a{};
+                a.x = x;
+                a.b = b;
+                a.yo = t32;
+                spmv<double>(*A, Prec::F64, EPI_RESID_MX, 0, nv, a, red_args(H), FinArgs<double>{});
+            }
+            {
+                SpmvArgs<float> a{};
+                a.x = t32;
+                a.y = bh;
+                spmv<float>(*Hh.levels[0].R, Prec::F32, EPI_AXY, 0, nv, a, red_args(H),
+                            FinArgs<float>{});
+            }
+            if (Hh.levels.size() == 2) {
+                lv32.coarse->apply(xh, true, bh, H);
+            } else {
+                mg_cycle_uniform<float>(Hh, lv32, nv, xh, true, bh, 1, cyc, H);
+                if (cyc == 2)
+                    mg_cycle_uniform<float>(Hh, lv32, nv, xh, false, bh, 1, 2, H);
+                else if (cyc == 1)
+                    mg_cycle_uniform<float>(Hh, lv32, nv, xh, false, bh, 1, 0, H);
+            }
+            {
+                SpmvArgs<float> a{};
+                a.x = xh;
+                a.y = t32;
+                spmv<float>(*Hh.levels[0].P, Prec::F32, EPI_AXY, 0, nv, a, red_args(H),
+                            FinArgs<float>{});
+            }
+            if (xzero)
+                fill<double>(x, A->n, nv, 0.0, H);
+            {
+                // x = 1*(double)temp32 + 1*x
+                EwArgs<double> c;
+                c.n = A->n;
+                c.z = x;
+                c.xo = t32;
+                ew<double>(EW_ADD_CONV, nv, c, H);
+            }
+            lv.post[0]->apply(x, false, b, H);
         }
-        if (xzero) {
-            // temp = 1*b (A*0 = 0)
-            EwArgs<T> a;
-            a.n = A->n;
-            a.z = tmp;
-            a.x = b;
-            a.y2 = tmp;
-            ew<T>(EW_COPY3, nv, a, H);
-        } else {
-            SpmvArgs<T> a{};
-            a.x = x;
-            a.y = tmp;
-            a.b = b;
-            spmv<T>(*A, this->vprec, EPI_RESID, 0, nv, a, red_args(H), FinArgs<T>{});
-        }
-        {
-            SpmvArgs<T> a{};
-            a.x = tmp;
-            a.y = bh;
-            spmv<T>(*L[lev].R, this->vprec, EPI_AXY, 0, nv, a, red_args(H), FinArgs<T>{});
-        }
-        if (lev + 1 == (int)L.size() - 1) {
-            coarse->apply(xh, true, bh, H);
-        } else {
-            mg_cycle(xh, true, bh, lev + 1, cyc, H);
-            if (cyc == 2)
-                mg_cycle(xh, false, bh, lev + 1, 2, H);
-            else if (cyc == 1)
-                mg_cycle(xh, false, bh, lev + 1, 0, H);
-        }
-        if (xzero)
-            fill<T>(x, A->n, nv, (T)0, H);
-        {
-            SpmvArgs<T> a{};
-            a.x = xh;
-            a.y = x;
-            spmv<T>(*L[lev].P, this->vprec, EPI_AXPY, 0, nv, a, red_args(H), FinArgs<T>{});
-        }
-        post[lev]->apply(x, false, b, H);
+    }
+    void cycle_once(T *x, bool xzero, const T *b, const int *H) {
+        if (mixed)
+            mg_cycle_mixed(x, xzero, b, cycle, H);
+        else
+            mg_cycle_uniform<T>(*hier, lv, this->nv, x, xzero, b, 0, cycle, H);
     }
     void apply(T *x, bool xzero, const T *b, const int *halt) override {
-        const int nv = this->nv;
         if (this->conv_check) {
-            state_init<T>(this->ks, this->cs, nv, this->hist_cap);
+            state_init<T>(this->ks, this->cs, this->nv, this->hist_cap);
             const int *H = &this->cs->halt;
             T *r = this->vec(0);
             resid_norm<T>(this, x, xzero, b, r, FIN_NORM_INIT, nullptr, true);
             bool xz = xzero;
             run_loop(this->max_iters, this->cs, [&]() {
-                mg_cycle(x, xz, b, 0, cycle, H);
+                cycle_once(x, xz, b, H);
                 xz = false;
                 resid_norm<T>(this, x, false, b, r, FIN_NORM, H, true);
             });
@@ -993,7 +1104,7 @@ class MultiGrid : public Solver<T> {
         }
         bool xz = xzero;
         for (int it = 0; it < this->max_iters; it++) {
-            mg_cycle(x, xz, b, 0, cycle, halt);
+            cycle_once(x, xz, b, halt);
             xz = false;
         }
     }

@@ -3,6 +3,7 @@
 // device-side halt flag that the host polls once per batch of iterations).
 #pragma once
 
+#include "amg.hpp"
 #include "params.hpp"
 #include "runtime.hpp"
 
@@ -90,6 +91,8 @@ int run_loop(int max_iters, ConvState *cs, const std::function<void()> &body);
 // Host classical AMG setup (Ruge-Stueben coarsening, direct interpolation, Galerkin
 // triple product) standing in for hypre's BoomerAMG setup.
 std::shared_ptr<MgHierarchy> build_amg(DevMatrix *A, const ParamList &mgprm);
+// Device copy of a host hierarchy whose level 0 is A0.
+std::shared_ptr<MgHierarchy> upload_hierarchy(DevMatrix *A0, const amg::Hierarchy &h);
 
 // Options of the execution engine (batch size of the early-exit polling).
 struct EngineCfg {

@@ -111,6 +111,17 @@ def lib():
             "xamgx_set_spmv_vpt": (ci, [ci, ci, ci]),
             "xamgx_matrix_set_eigs": (ci, [sz, cd, cd]),
             "xamgx_matrix_get_eigs": (ci, [sz, vp, vp]),
+            "xamgx_hierarchy_build": (ci, [sz, sz, psz]),
+            "xamgx_hierarchy_build_csr": (ci, [sz, C.c_uint64, vp, vp, vp, psz]),
+            "xamgx_hierarchy_create": (ci, [psz]),
+            "xamgx_hierarchy_add_level": (ci, [sz, C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp, vp]),
+            "xamgx_hierarchy_num_levels": (ci, [sz, vp]),
+            "xamgx_hierarchy_level_size": (ci, [sz, ci, vp]),
+            "xamgx_hierarchy_level_get": (ci, [sz, ci, vp, vp, vp, vp, vp, vp]),
+            "xamgx_hierarchy_set_eigs": (ci, [sz, ci, cd, cd]),
+            "xamgx_hierarchy_get_eigs": (ci, [sz, ci, vp, vp]),
+            "xamgx_matrix_set_hierarchy": (ci, [sz, sz]),
+            "xamgx_hierarchy_destroy": (ci, [sz]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -264,6 +275,84 @@ class Matrix:
         check(lib().XAMG_matrix_destroy(self.id))
 
 
+def _csr_arrays(row, col, val):
+    return (np.ascontiguousarray(row, dtype=np.int32), np.ascontiguousarray(col, dtype=np.int32),
+            np.ascontiguousarray(val, dtype=np.float64))
+
+
+class Hierarchy:
+    """Multigrid levels (xamgx_hierarchy_*): the host classical AMG setup run on a
+    matrix or CSR arrays with the MultiGrid keys of a parameter dictionary, or levels
+    imported one by one.  level(l) returns {"n", "A": (row, col, val) for l >= 1,
+    "P": (row, col, val, ncoarse) for l < levels-1}."""
+
+    def __init__(self, hid):
+        self.id = hid
+
+    @classmethod
+    def build(cls, matrix, params):
+        h = C.c_size_t()
+        check(lib().xamgx_hierarchy_build(matrix.id, params.id, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def build_csr(cls, params, row, col, val):
+        row, col, val = _csr_arrays(row, col, val)
+        h = C.c_size_t()
+        check(lib().xamgx_hierarchy_build_csr(params.id, len(row) - 1, _p(row), _p(col), _p(val),
+                                              C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def create(cls):
+        return cls(_new(lib().xamgx_hierarchy_create))
+
+    def add_level(self, P, A):
+        """P = (row, col, val, ncoarse) of the interpolation, A = (row, col, val)."""
+        pr, pc, pv = _csr_arrays(*P[:3])
+        ar, ac, av = _csr_arrays(*A)
+        check(lib().xamgx_hierarchy_add_level(self.id, len(pr) - 1, P[3], _p(pr), _p(pc), _p(pv),
+                                              _p(ar), _p(ac), _p(av)))
+
+    def levels(self):
+        n = C.c_int()
+        check(lib().xamgx_hierarchy_num_levels(self.id, C.byref(n)))
+        return n.value
+
+    def level(self, l):
+        sz = np.zeros(4, dtype=np.uint64)
+        check(lib().xamgx_hierarchy_level_size(self.id, l, _p(sz)))
+        n, annz, nc, pnnz = (int(v) for v in sz)
+        out = {"n": n}
+        ar = ac = av = pr = pc = pv = None
+        if l > 0:
+            ar, ac, av = np.zeros(n + 1, np.int32), np.zeros(annz, np.int32), np.zeros(annz)
+        if l + 1 < self.levels():
+            pr, pc, pv = np.zeros(n + 1, np.int32), np.zeros(pnnz, np.int32), np.zeros(pnnz)
+        nul = lambda a: None if a is None else _p(a)
+        check(lib().xamgx_hierarchy_level_get(self.id, l, nul(ar), nul(ac), nul(av),
+                                              nul(pr), nul(pc), nul(pv)))
+        if ar is not None:
+            out["A"] = (ar, ac, av)
+        if pr is not None:
+            out["P"] = (pr, pc, pv, nc)
+        return out
+
+    def eigs(self, l):
+        lo, hi = C.c_double(), C.c_double()
+        check(lib().xamgx_hierarchy_get_eigs(self.id, l, C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def set_eigs(self, l, lo, hi):
+        check(lib().xamgx_hierarchy_set_eigs(self.id, l, lo, hi))
+
+    def attach(self, matrix):
+        check(lib().xamgx_matrix_set_hierarchy(matrix.id, self.id))
+
+    def destroy(self):
+        check(lib().xamgx_hierarchy_destroy(self.id))
+
+
 class Solver:
     def __init__(self, matrix, params, x, b):
         self.nv = x.nv
@@ -375,12 +464,15 @@ def device_info():
     return buf.value.decode()
 
 
-def solve_system(row, col, val, b, nv, params, x0=None, eigs=None, keep=False):
+def solve_system(row, col, val, b, nv, params, x0=None, eigs=None, keep=False, hierarchy=None):
     """Upload A and b, build the solver from `params` (role -> {key: value}), solve.
+    `hierarchy` (a Hierarchy) is attached to the matrix for MultiGrid solvers.
     Returns a dict with x, iterations, convergence flags, residuals, history, time."""
     A = Matrix(row, col, val)
     if eigs is not None:
         A.set_eigs(*eigs)
+    if hierarchy is not None:
+        hierarchy.attach(A)
     n = A.n
     x = Vector(n, nv)
     bv = Vector(n, nv)
