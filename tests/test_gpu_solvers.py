@@ -149,6 +149,30 @@ def test_zero_rhs_converges_initially(xamg):
     assert gpu["converged"] == [True]
 
 
+def test_zero_rhs_beside_nonzero_stays_finite(xamg):
+    """A zero RHS beside non-zero ones: its recurrence breaks down (rho = 0, alpha = 0/0)
+    but it is converged from the start, so its masked update is exactly 0 and x stays 0,
+    while the other RHS follow the oracle.  (The reference forms the masked coefficient
+    as 0 * NaN and returns NaN for that RHS; the oracle restates that.)"""
+    nx, nv = 12, 4
+    row, col, val = xamg.poisson7(nx)
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 6).reshape(n, nv)
+    b[:, 2] = 0.0
+    b = b.reshape(-1)
+    params = _params("PBiCGStab", {"method": "Jacobi"})
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    X = gpu["x"].reshape(n, nv)
+    assert np.all(X[:, 2] == 0.0)
+    assert np.all(np.isfinite(X))
+    orc = O.solve(params, O.csr(row, col, val), b, nv)
+    assert np.isnan(orc["x"].reshape(n, nv)[:, 2]).all()
+    keep = [0, 1, 3]
+    assert abs(gpu["iters"] - orc["iters"]) <= 2
+    xo = orc["x"].reshape(n, nv)[:, keep]
+    assert np.linalg.norm(X[:, keep] - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("inner", ["BiCGStab", "PBiCGStab"])
 def test_ir_fp32_inner_fp64_outer(xamg, inner):
     """IterativeRefinement with an FP32 inner solver reaches 1e-10 verified in FP64."""

@@ -58,6 +58,7 @@ struct ConvState {
     // continued refinement (FP32 Krylov, FP64 residual replacement)
     int replace;            // set when the FP64 true residual must be recomputed
     int replaced;           // set by that recomputation for the following r update
+    int restart;            // FP32 recurrence broke down: restart it from the FP64 residual
     double maxres[kMaxNV];  // squared residual at the last FP64 check
 };
 

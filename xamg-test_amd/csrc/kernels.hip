@@ -40,6 +40,15 @@ __device__ __forceinline__ int logical_block(int b, int G, int remap) {
     return b;
 }
 
+// Masked coefficient of a per-RHS solution update (pbicgstab.inl:163-165 forms it as
+// mask * coefficient).  A converged RHS gets exactly 0 even when its own recurrence has
+// since broken down (0/0 once its residual underflows), where the product would give
+// NaN and poison that RHS's solution; for finite coefficients the two agree bit for bit.
+template <typename F>
+__device__ inline F mask_coef(F m, F c) {
+    return m != (F)0 ? m * c : (F)0;
+}
+
 // Device-side convergence check (convergence.h:209-262): per-RHS residual update,
 // history row, comparison in the iteration precision F, mask update.
 template <typename F>
@@ -61,17 +70,23 @@ __device__ void conv_check(const FinArgs<F> &f, const F *rn, int nv, int initial
             f.hist[(int64_t)cs->nhist * nv + l] = cs->status[l] ? -1.0 : cs->res[l];
         cs->nhist++;
     }
-    int flag = 1;
+    // The reference tests `abs_res > tol` and takes every other outcome as converged,
+    // NaN included; here a NaN residual ends the solve with that RHS not converged.
+    int flag = 1, bad = 0;
     for (int l = 0; l < nv; l++) {
         if (cs->status[l])
             continue;
         F ar = (F)sqrt(cs->res[l]);
-        if (ar > (F)cs->tol[l])
+        if (ar > (F)cs->tol[l]) {
             flag = 0;
-        else
+        } else if (ar == ar) {
             cs->status[l] = 1;
+        } else {
+            flag = 0;
+            bad = 1;
+        }
     }
-    if (flag)
+    if (flag || bad)
         cs->done = 1;
     else if (f.iconv != nullptr)
         for (int l = 0; l < nv; l++)
@@ -111,8 +126,8 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
             F om = s2 / s3;
             ks->omega[l] = om;
             ks->nomega[l] = om * (F)-1.0;
-            ks->ac[l] = ks->iconv[l] * ks->alpha[l];
-            ks->oc[l] = ks->iconv[l] * om;
+            ks->ac[l] = mask_coef(ks->iconv[l], ks->alpha[l]);
+            ks->oc[l] = mask_coef(ks->iconv[l], om);
             F res = om * s2;
             res = (F)1.0 * s4 + (F)(-1.0) * res;
             v[l] = res;
@@ -157,7 +172,7 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
             k32->rho[l] = (float)sums[l];
             cs->maxres[l] = sums[l];
         }
-        cs->replace = cs->replaced = 0;
+        cs->replace = cs->replaced = cs->restart = 0;
         conv_check(f, v, nv, 1);
         cs->halt = cs->done;
         if (!cs->done)
@@ -167,23 +182,34 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
         break;
     }
     case FIN_MX_OMEGA: {
-        int req = 0;
+        int req = 0, brk = 0;
         for (int l = 0; l < nv; l++) {
             F s2 = (F)sums[l], s3 = (F)sums[nv + l], s4 = (F)sums[2 * nv + l];
             F om = s2 / s3;
             ks->omega[l] = om;
             ks->nomega[l] = om * (F)-1.0;
-            ks->ac[l] = ks->iconv[l] * ks->alpha[l];
-            ks->oc[l] = ks->iconv[l] * om;
+            ks->ac[l] = mask_coef(ks->iconv[l], ks->alpha[l]);
+            ks->oc[l] = mask_coef(ks->iconv[l], om);
             F res = om * s2;
             res = (F)1.0 * s4 + (F)(-1.0) * res;
             v[l] = res;
             if (!cs->status[l]) {
                 const double rr = (double)res;
+                // breakdown of the FP32 recurrence (rho or omega at 0, or not finite):
+                // skip this update and restart from the FP64 residual
+                if (!(isfinite(om) && om != (F)0 && isfinite(ks->alpha[l]) && isfinite(res))) {
+                    ks->ac[l] = ks->oc[l] = (F)0;
+                    brk = 1;
+                    continue;
+                }
                 cs->res[l] = rr;
                 if (rr <= cs->tol[l] * cs->tol[l] || rr <= f.delta2 * cs->maxres[l])
                     req = 1;
             }
+        }
+        if (brk) {
+            cs->restart = 1;
+            req = 1;
         }
         if (f.hist != nullptr && cs->nhist < cs->hist_cap) {
             for (int l = 0; l < nv; l++)
@@ -197,7 +223,7 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
     case FIN_MX_REPL: {
         auto *k32 = (KrylovScalars<float> *)f.aux;
         cs->sub_iter++;
-        int flag = 1;
+        int flag = 1, bad = 0;
         for (int l = 0; l < nv; l++) {
             if (cs->status[l])
                 continue;
@@ -207,8 +233,10 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
                 cs->status[l] = 1;
             else
                 flag = 0;
+            if (sums[l] != sums[l])
+                bad = 1;
         }
-        if (flag)
+        if (flag || bad)
             cs->done = 1;
         else
             for (int l = 0; l < nv; l++)
@@ -487,10 +515,12 @@ __global__ void __launch_bounds__(kBlock)
                 for (int v = 0; v < VPT; v++)
                     o32[v] = (float)out[v];
                 st_vec<float, VPT>(a.yo + e, o32);
-                if (a.yo2 != nullptr)
-                    st_vec<float, VPT>(a.yo2 + e, o32);
-                if (a.yo3 != nullptr)
-                    st_vec<float, VPT>(a.yo3 + e, o32);
+                if (a.copy_if == nullptr || *a.copy_if) {
+                    if (a.yo2 != nullptr)
+                        st_vec<float, VPT>(a.yo2 + e, o32);
+                    if (a.yo3 != nullptr)
+                        st_vec<float, VPT>(a.yo3 + e, o32);
+                }
             } else {
                 st_vec<T, VPT>(a.y + e, out);
                 if (a.y2 != nullptr)
@@ -552,6 +582,11 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
     const T ca = a.a != nullptr ? a.a[l] : (T)1.0;
     const T cb = a.b != nullptr ? a.b[l] : (T)1.0;
     const T cc = a.c != nullptr ? a.c[l] : (T)1.0;
+    bool frozen = false;
+    if constexpr (OP == EW_AXPBYPCZ)
+        frozen = a.freeze_zero && ca == (T)0 && cb == (T)0;
+    else if constexpr (OP == EW_MX_XUPD)
+        frozen = a.freeze_zero && a.ao[l] == 0.0f && a.bo[l] == 0.0f;
     double dacc[ND > 0 ? ND : 1][1];
 #pragma unroll
     for (int k = 0; k < (ND > 0 ? ND : 1); k++)
@@ -563,7 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
         } else if constexpr (OP == EW_AXPBY_Z) {
             out = keep ? a.z[e] : ca * a.x[e] + cb * a.y[e];
         } else if constexpr (OP == EW_AXPBYPCZ) {
-            out = ca * a.x[e] + cb * a.y[e] + cc * a.z[e];
+            out = (frozen || keep) ? a.z[e] : ca * a.x[e] + cb * a.y[e] + cc * a.z[e];
         } else if constexpr (OP == EW_JACOBI0) {
             const int64_t i = e / NV;
             out = (ca * a.d[i]) * a.x[e];
@@ -592,8 +627,9 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
         } else if constexpr (OP == EW_JAC_UPD) {
             out = (ca * cc) * (a.y[e] * a.d[e / NV]) + (T)1.0 * a.z[e];
         } else { // EW_MX_XUPD: FP64 accumulation of FP32 directions
-            out = (T)a.ao[l] * (T)((const float *)a.xo)[e] +
-                  (T)a.bo[l] * (T)((const float *)a.yo)[e] + (T)1.0 * a.z[e];
+            out = frozen ? a.z[e]
+                         : (T)a.ao[l] * (T)((const float *)a.xo)[e] +
+                               (T)a.bo[l] * (T)((const float *)a.yo)[e] + (T)1.0 * a.z[e];
         }
         a.z[e] = out;
         if constexpr (ND == 1) {

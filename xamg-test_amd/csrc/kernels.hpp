@@ -103,6 +103,7 @@ struct SpmvArgs {
     T *y2 = nullptr, *y3 = nullptr; // extra copies of y (initial residual: r, p)
     // EPI_RESID_MX: FP32 copies of the FP64 residual (y is not written)
     float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr;
+    const int *copy_if = nullptr; // EPI_RESID_MX: yo2 / yo3 written only while *copy_if != 0
     // launch configuration chosen by the autotuner (0 / -1: global defaults)
     int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1;
 };
@@ -120,7 +121,11 @@ struct EwArgs {
     T *y2 = nullptr, *y3 = nullptr;                   // COPY3 extra outputs
     const void *yo = nullptr;                         // second other-precision input
     const float *ao = nullptr, *bo = nullptr;         // FP32 coefficients (EW_MX_XUPD)
-    const int *keep_if = nullptr; // EW_AXPBY_Z: leave z unchanged while *keep_if != 0
+    const int *keep_if = nullptr; // AXPBY_Z / AXPBYPCZ: leave z unchanged while *keep_if != 0
+    // AXPBYPCZ / MX_XUPD: leave z unchanged for an RHS whose coefficients a and b are
+    // both exactly 0 (the masked solution update of a converged RHS keeps its solution
+    // even when that RHS's directions have broken down to NaN)
+    int freeze_zero = 0;
 };
 
 struct LaunchCfg {

@@ -569,6 +569,7 @@ class Krylov : public Solver<T> {
                 a.a = ks->ac;
                 a.y = s_;
                 a.b = ks->oc;
+                a.freeze_zero = 1;
                 ew<T>(EW_AXPBYPCZ, nv, a, H);
             }
             { // r = s - omega t, rho_n = (r, r0), beta
@@ -800,7 +801,9 @@ class ContinuedRefine : public Solver<double> {
                 a.x = p_;
                 a.y = v;
                 a.w = r0;
-                spmv<float>(*A, Prec::F32, EPI_AXY, 1, nv, a, red_args(H), fin32(FIN_ALPHA));
+                FinArgs<float> fa = fin32(FIN_ALPHA);
+                fa.clear_on_fin = &cs->restart;
+                spmv<float>(*A, Prec::F32, EPI_AXY, 1, nv, a, red_args(H), fa);
             }
             {
                 EwArgs<float> a;
@@ -828,13 +831,18 @@ class ContinuedRefine : public Solver<double> {
                 a.yo = s_;
                 a.ao = k32->ac;
                 a.bo = k32->oc;
+                a.freeze_zero = 1;
                 ew<double>(EW_MX_XUPD, nv, a, H);
             }
-            { // FP64 residual replacement when requested
+            { // FP64 residual replacement when requested; after a breakdown it also
+              // restarts the recurrence (r0 = p = r)
                 SpmvArgs<double> a{};
                 a.x = x;
                 a.b = b;
                 a.yo = r;
+                a.yo2 = r0;
+                a.yo3 = p;
+                a.copy_if = &cs->restart;
                 RedArgs ra = red_args(H);
                 ra.gate = &cs->replace;
                 spmv<double>(*A, Prec::F64, EPI_RESID_MX, 1, nv, a, ra, fin64(FIN_MX_REPL));
@@ -855,7 +863,7 @@ class ContinuedRefine : public Solver<double> {
                 fa.clear_on_fin = &cs->replaced;
                 ew<float>(EW_AXPBY_Z, 1, nv, a, ra, fa);
             }
-            {
+            { // p = r + beta p - beta omega v (kept as r after a restart)
                 EwArgs<float> a;
                 a.n = A->n;
                 a.z = p;
@@ -863,6 +871,7 @@ class ContinuedRefine : public Solver<double> {
                 a.y = v;
                 a.b = k32->nbo;
                 a.c = k32->beta;
+                a.keep_if = &cs->restart;
                 ew<float>(EW_AXPBYPCZ, nv, a, H);
             }
         };
