@@ -33,7 +33,7 @@ def spmv_bytes(n, nnz, nv, s_val, s_vec):
     return 4 * (n + 1) + nnz * (4 + s_val) + 2 * n * nv * s_vec
 
 
-def solver_params(kind, mixed, inner_rtol, tol, refine="continued", step=1e-2):
+def solver_params(kind, mixed, inner_rtol, tol, refine="continued", step=1e-2, extra=None):
     if kind == "cheb4":
         prec = {"method": "Chebyshev", "polynomial_order": 4, "spectrum_fraction": 0.3}
         method = "PBiCGStab"
@@ -50,7 +50,7 @@ def solver_params(kind, mixed, inner_rtol, tol, refine="continued", step=1e-2):
         p = {"meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
                              "max_iters": 50, "rel_tolerance": tol, "abs_tolerance": 0.0,
                              "convergence_details": "check-L2", "refinement": refine,
-                             "replace_step": step},
+                             "replace_step": step, **(extra or {})},
              "solver": {"method": method, "rel_tolerance": inner_rtol, "abs_tolerance": 0.0,
                         "max_iters": 5000, "convergence_details": "check-L2"}}
     if prec is not None:
@@ -99,6 +99,11 @@ def main():
                     help="inner tolerance of the restarted refinement (reference IR)")
     ap.add_argument("--refine", default="continued", choices=["continued", "restart"])
     ap.add_argument("--replace-step", type=float, default=1e-2)
+    ap.add_argument("--restart-threshold", type=float, default=None,
+                    help="continued refinement: restart when |(r,r0)| < t ||r|| ||r0||")
+    ap.add_argument("--restart-gap", type=float, default=None,
+                    help="continued refinement: restart when the true residual exceeds the "
+                         "recurrence residual by this factor (0: off)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -141,8 +146,13 @@ def main():
     bv.upload(b)
 
     def make(mixed):
+        extra = {}
+        if args.restart_threshold is not None:
+            extra["restart_threshold"] = args.restart_threshold
+        if args.restart_gap is not None:
+            extra["restart_gap"] = args.restart_gap
         P = X.Params(solver_params(args.solver, mixed, args.inner_rtol, args.tol, args.refine,
-                                   args.replace_step))
+                                   args.replace_step, extra))
         P.set_default()
         return X.Solver(A, P, xv, bv)
 

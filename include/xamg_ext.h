@@ -113,6 +113,46 @@ int xamgx_hierarchy_get_eigs(XAMG_hierarchy_id hid, int level, double *lo, doubl
 int xamgx_matrix_set_hierarchy(XAMG_matrix_id mid, XAMG_hierarchy_id hid);
 int xamgx_hierarchy_destroy(XAMG_hierarchy_id hid);
 
+/* ---- batches of independent systems ----
+ * Many independent systems of uneven sizes, each with its own nv (1..4), solved
+ * together: the reference solves them one by one (one solver per system, NV fixed at
+ * compile time); here one launch per iteration step covers all of them and systems
+ * drop out as they converge.  The matrix is the concatenated block-diagonal CSR with
+ * global indices (int, as XAMG_csr_matrix_fill); sys_rows[k] is the first row of
+ * system k (nsys + 1 entries, sys_rows[0] = 0).  Vectors concatenate each system's
+ * [n_k][nv_k] row-interleaved block in system order; pass them as device vectors of
+ * sum_k n_k nv_k rows and nv = 1. */
+typedef size_t XAMG_batch_id;
+int xamgx_batch_create(int nsys, const int *sys_rows, const int *nv, const int *row, const int *col,
+                       const double *val, XAMG_batch_id *bid);
+/* Solver from a parameter dictionary: "solver" BiCGStab or PBiCGStab with a Jacobi
+ * "preconditioner" (max_iters sweeps, relax_factor) in FP64 (pbicgstab.inl semantics),
+ * or under a "meta_solver" IterativeRefinement with subsolver_reduced_precision: FP32
+ * values and recurrences, FP64 solution and residual replacement, convergence on the
+ * FP64 residual (replace_step as for the single-system solver). */
+int xamgx_batch_setup(XAMG_batch_id bid, XAMG_param_id pid);
+int xamgx_batch_solve(XAMG_batch_id bid, XAMG_vector_id x, XAMG_vector_id b);
+/* Per system: Krylov iterations and FP64 residual checks (mixed precision); per RHS in
+ * vector order: convergence flags.  NULL skips an output. */
+int xamgx_batch_get_info(XAMG_batch_id bid, int *iters, int *checks, int *converged);
+/* FP64 relative residuals ||b - A x|| / ||b|| per RHS, in vector order. */
+int xamgx_batch_residuals(XAMG_batch_id bid, XAMG_vector_id x, XAMG_vector_id b, double *relres);
+/* Device time (ms) of the last batch solve. */
+int xamgx_batch_last_ms(XAMG_batch_id bid, double *ms);
+/* Workgroups of the persistent grid, rows per tile, iterations per host poll, hipGraph
+ * replay (0/1); values <= 0 (< 0 for use_graph) keep the current setting. */
+int xamgx_batch_set_launch(int grid, int tile_rows, int batch, int use_graph);
+int xamgx_batch_destroy(XAMG_batch_id bid);
+/* Workload generator: nsys 2D 5-point variable-coefficient systems, side
+ * round(smin (smax/smin)^U_k) and nv_k = 1 + floor(maxnv V_k) with U_k, V_k the
+ * pseudo_rand values at seed*2^27 + 2k, + 2k + 1; system k's coefficients are those of
+ * xamgx_gen_var5(side_k, seed + 7919 (k + 1)). */
+int xamgx_gen_batch_var5_layout(int nsys, uint32_t seed, int smin, int smax, int maxnv, int *side,
+                                int *nv);
+int64_t xamgx_gen_batch_var5_nnz(int nsys, const int *side);
+int xamgx_gen_batch_var5(int nsys, const int *side, uint32_t seed, int *sys_rows, int *row, int *col,
+                         double *val);
+
 #ifdef __cplusplus
 }
 #endif

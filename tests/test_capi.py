@@ -164,3 +164,26 @@ def test_param_dictionary_defaults_without_gpu():
         P.change("solver", "no_such_key", 1.0)
     P.change("solver", "no_such_key", 1.0, forced=True)
     P.destroy()
+
+
+def test_batch_generator_blocks():
+    """The batch generator concatenates var5 systems: layout from the pseudo_rand stream,
+    each diagonal block equal to xamgx_gen_var5(side_k, seed + 7919 (k + 1)), no entry
+    outside its block."""
+    import pyxamg as X
+    nsys, seed = 40, 5
+    side, nv, sr, row, col, val = X.batch_var5(nsys, seed, smin=3, smax=40)
+    assert side.min() >= 3 and side.max() <= 40 and set(nv) <= {1, 2, 3, 4}
+    assert np.array_equal(np.diff(sr), side.astype(np.int64) ** 2)
+    for k in range(nsys):
+        r0, r1 = sr[k], sr[k + 1]
+        lr, lc, lv = X.var5(int(side[k]), (seed + 7919 * (k + 1)) & 0xffffffff)
+        assert np.array_equal(row[r0:r1 + 1] - row[r0], lr)
+        seg = slice(row[r0], row[r1])
+        assert np.array_equal(col[seg] - r0, lc)
+        assert np.array_equal(val[seg], lv)
+    # the stream: U_k = pseudo_rand(seed*2^27 + 2k) sets the side
+    import oracle as O
+    u = [O.pseudo_rand_d((seed * (1 << 27) + 2 * k) & 0xffffffff) for k in range(nsys)]
+    expect = np.rint(3 * (40 / 3) ** np.array(u)).astype(int)
+    assert np.array_equal(side, expect)

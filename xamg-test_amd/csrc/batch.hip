@@ -1,0 +1,1088 @@
+// Batched Krylov solves over a concatenated block-diagonal CSR (see batch.hpp).
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace xamg {
+
+BatchCfg &batch_cfg() {
+    static BatchCfg c;
+    return c;
+}
+
+namespace {
+
+constexpr int BT = 256;      // threads per workgroup
+constexpr int PSTRIDE = kBatchDots * kBatchMaxNV; // partial sums per tile
+
+template <typename T>
+struct BSpmvArgs {
+    const void *val = nullptr; // V[nnz]
+    const T *x = nullptr;
+    T *y = nullptr, *y2 = nullptr, *y3 = nullptr;
+    const T *b = nullptr;
+    const T *xin = nullptr;
+    const T *d = nullptr;     // inverse diagonal per global row
+    const T *w = nullptr;     // dot partner
+    const T *wcoef = nullptr; // JACOBI relax factors, per (system, RHS)
+    float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr; // EPI_RESID_MX outputs
+    const int *gate = nullptr;    // per system: tiles run only while set
+    const int *copy_if = nullptr; // per system: yo2 / yo3 written only while set
+};
+
+template <typename T>
+struct BEwArgs {
+    T *z = nullptr;
+    const T *x = nullptr, *y = nullptr;
+    const T *ca = nullptr, *cb = nullptr, *cc = nullptr; // per (system, RHS); null = 1
+    const T *d = nullptr;
+    T c0 = 0;
+    const void *xo = nullptr, *yo = nullptr; // FP32 inputs of EW_MX_XUPD
+    const float *ao = nullptr, *bo = nullptr; // FP32 coefficients of EW_MX_XUPD
+    T *y2 = nullptr, *y3 = nullptr;
+    const T *w = nullptr;
+    const int *keep = nullptr; // per system: z unchanged while set
+    // per system, while set: AXPBY_Z also stores z into y2 and dots z with itself;
+    // AXPBYPCZ sets z = x (restart of the recurrence from its residual)
+    const int *copy = nullptr;
+    int freeze_zero = 0;       // AXPBYPCZ / MX_XUPD: z unchanged where a and b are both 0
+};
+
+struct BFinArgs {
+    int fin = FIN_NONE;
+    int nd = 0;
+    double abs_tol = 0, rel_tol = 0, delta2 = 0;
+    double theta = 0; // FIN_BETA with (r, r0), (r, r), (r0, r0): restart threshold
+    double gap2 = 0;  // FIN_MX_REPL: restart when true > gap2 * recurrence residual^2
+    int max_iters = 1;
+    const int *gate = nullptr; // per system
+};
+
+// Sum of the per-thread accumulators of one tile over the workgroup, in a fixed order,
+// stored as the tile's partial sums by thread 0.
+template <int ND>
+__device__ void tile_partial(double (&acc)[ND][kBatchMaxNV], double *out) {
+    __shared__ double lds[BT / 64][ND][kBatchMaxNV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < ND; k++)
+#pragma unroll
+        for (int l = 0; l < kBatchMaxNV; l++) {
+            double v = acc[k][l];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                v += __shfl_xor(v, off, 64);
+            if (lane == 0)
+                lds[wave][k][l] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < ND; k++)
+            for (int l = 0; l < kBatchMaxNV; l++) {
+                double v = lds[0][k][l];
+                for (int q = 1; q < BT / 64; q++)
+                    v += lds[q][k][l];
+                out[k * kBatchMaxNV + l] = v;
+            }
+    __syncthreads();
+}
+
+// y = epilogue(A x) over the tiles of the running systems; csr_matvec.inl:213-231 row
+// sums in ascending column order, x * val.
+template <typename V, typename T, int EPI, int ND>
+__global__ void __launch_bounds__(BT)
+    kb_spmv(BLayout L, const int *halt, BSpmvArgs<T> a, double *part) {
+    const V *val = (const V *)a.val;
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int s = L.tsys[t];
+        if ((halt != nullptr && halt[s]) || (a.gate != nullptr && !a.gate[s]))
+            continue;
+        const int nv = L.nv[s];
+        const int64_t vb = L.voff[s] - (int64_t)L.srow[s] * nv;
+        const uint32_t r0 = L.trow[t], r1 = L.trow[t + 1];
+        const bool cp = a.copy_if == nullptr || a.copy_if[s];
+        constexpr int NDX = ND > 0 ? ND : 1;
+        double dacc[NDX][kBatchMaxNV];
+#pragma unroll
+        for (int k = 0; k < NDX; k++)
+#pragma unroll
+            for (int l = 0; l < kBatchMaxNV; l++)
+                dacc[k][l] = 0.0;
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += BT) {
+            T acc[kBatchMaxNV] = {(T)0, (T)0, (T)0, (T)0};
+            const uint32_t j1 = L.row[i + 1];
+            for (uint32_t j = L.row[i]; j < j1; j++) {
+                const T v = (T)val[j];
+                const T *xp = a.x + vb + (int64_t)L.col[j] * nv;
+#pragma unroll
+                for (int l = 0; l < kBatchMaxNV; l++)
+                    if (l < nv)
+                        acc[l] = acc[l] + xp[l] * v;
+            }
+            const int64_t e = vb + (int64_t)i * nv;
+#pragma unroll
+            for (int l = 0; l < kBatchMaxNV; l++) {
+                if (l >= nv)
+                    continue;
+                T out;
+                if constexpr (EPI == EPI_AXY) {
+                    out = acc[l];
+                } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
+                    out = (T)1.0 * a.b[e + l] + (T)(-1.0) * acc[l];
+                } else { // EPI_JACOBI: x_new = x + (w d)(b - A x)
+                    out = a.xin[e + l] + (a.wcoef[s * kBatchMaxNV + l] * a.d[i]) *
+                                             (a.b[e + l] - acc[l]);
+                }
+                if constexpr (EPI == EPI_RESID_MX) {
+                    const float o = (float)out;
+                    a.yo[e + l] = o;
+                    if (cp) {
+                        if (a.yo2 != nullptr)
+                            a.yo2[e + l] = o;
+                        if (a.yo3 != nullptr)
+                            a.yo3[e + l] = o;
+                    }
+                } else {
+                    a.y[e + l] = out;
+                    if (a.y2 != nullptr)
+                        a.y2[e + l] = out;
+                    if (a.y3 != nullptr)
+                        a.y3[e + l] = out;
+                }
+                if constexpr (ND == 1) {
+                    const double wv = a.w != nullptr ? (double)a.w[e + l] : (double)out;
+                    dacc[0][l] += (double)out * wv;
+                } else if constexpr (ND == 3) {
+                    const double o = (double)out, wv = (double)a.w[e + l];
+                    dacc[0][l] += o * wv;
+                    dacc[1][l] += o * o;
+                    dacc[2][l] += wv * wv;
+                }
+            }
+        }
+        if constexpr (ND > 0)
+            tile_partial<ND>(dacc, part + (int64_t)t * PSTRIDE);
+    }
+}
+
+// Element-wise updates over the tiles of the running systems (blas.inl / blas_ext.inl
+// operation order, per-(system, RHS) coefficients).
+template <typename T, int OP, int ND>
+__global__ void __launch_bounds__(BT)
+    kb_ew(BLayout L, const int *halt, BEwArgs<T> a, double *part) {
+    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+        const int s = L.tsys[t];
+        if (halt != nullptr && halt[s])
+            continue;
+        const bool keep = a.keep != nullptr && a.keep[s];
+        const bool copy = a.copy != nullptr && a.copy[s];
+        const int nv = L.nv[s];
+        const int64_t vb = L.voff[s] - (int64_t)L.srow[s] * nv;
+        const uint32_t r0 = L.trow[t], r1 = L.trow[t + 1];
+        T ca[kBatchMaxNV], cb[kBatchMaxNV], cc[kBatchMaxNV];
+        bool frozen[kBatchMaxNV];
+#pragma unroll
+        for (int l = 0; l < kBatchMaxNV; l++) {
+            const int q = s * kBatchMaxNV + l;
+            ca[l] = a.ca != nullptr ? a.ca[q] : (T)1.0;
+            cb[l] = a.cb != nullptr ? a.cb[q] : (T)1.0;
+            cc[l] = a.cc != nullptr ? a.cc[q] : (T)1.0;
+            frozen[l] = false;
+            if constexpr (OP == EW_AXPBYPCZ)
+                frozen[l] = a.freeze_zero && ca[l] == (T)0 && cb[l] == (T)0;
+            else if constexpr (OP == EW_MX_XUPD)
+                frozen[l] = a.freeze_zero && a.ao[q] == 0.0f && a.bo[q] == 0.0f;
+        }
+        constexpr int NDX = ND > 0 ? ND : 1;
+        double dacc[NDX][kBatchMaxNV];
+#pragma unroll
+        for (int k = 0; k < NDX; k++)
+#pragma unroll
+            for (int l = 0; l < kBatchMaxNV; l++)
+                dacc[k][l] = 0.0;
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += BT) {
+            const int64_t e0 = vb + (int64_t)i * nv;
+#pragma unroll
+            for (int l = 0; l < kBatchMaxNV; l++) {
+                if (l >= nv)
+                    continue;
+                const int64_t e = e0 + l;
+                T out;
+                if constexpr (OP == EW_FILL) {
+                    out = a.c0;
+                } else if constexpr (OP == EW_AXPBY_Z) {
+                    out = keep ? a.z[e] : ca[l] * a.x[e] + cb[l] * a.y[e];
+                    if (copy && a.y2 != nullptr)
+                        a.y2[e] = out;
+                } else if constexpr (OP == EW_AXPBYPCZ) {
+                    if (copy)
+                        out = a.x[e];
+                    else
+                        out = (keep || frozen[l])
+                                  ? a.z[e]
+                                  : ca[l] * a.x[e] + cb[l] * a.y[e] + cc[l] * a.z[e];
+                } else if constexpr (OP == EW_JACOBI0) {
+                    out = (ca[l] * a.d[i]) * a.x[e];
+                } else if constexpr (OP == EW_COPY3) {
+                    out = (T)1.0 * a.x[e];
+                    if (a.y2 != nullptr)
+                        a.y2[e] = out;
+                    if (a.y3 != nullptr)
+                        a.y3[e] = out;
+                } else { // EW_MX_XUPD
+                    const int q = s * kBatchMaxNV + l;
+                    out = frozen[l] ? a.z[e]
+                                    : (T)a.ao[q] * (T)((const float *)a.xo)[e] +
+                                          (T)a.bo[q] * (T)((const float *)a.yo)[e] + (T)1.0 * a.z[e];
+                }
+                a.z[e] = out;
+                if constexpr (ND == 1) {
+                    const double wv = (a.w != nullptr && !copy) ? (double)a.w[e] : (double)out;
+                    dacc[0][l] += (double)out * wv;
+                } else if constexpr (ND == 3) {
+                    const double o = (double)out, wv = copy ? o : (double)a.w[e];
+                    dacc[0][l] += o * wv;
+                    dacc[1][l] += o * o;
+                    dacc[2][l] += wv * wv;
+                }
+            }
+        }
+        if constexpr (ND > 0)
+            tile_partial<ND>(dacc, part + (int64_t)t * PSTRIDE);
+    }
+}
+
+// Per-system convergence test (convergence.h:209-262 semantics, NaN residuals end the
+// system unconverged).  Returns true when the system is done.
+template <typename F>
+__device__ bool bconv_check(BState<F> S, int ns, int s, int nv, const F *rn, bool initial,
+                            double abs_tol, double rel_tol, F *iconv) {
+    double *res0 = S.cv + BC_RES0 * ns * kBatchMaxNV, *res = S.cv + BC_RES * ns * kBatchMaxNV,
+           *tol = S.cv + BC_TOL * ns * kBatchMaxNV;
+    const int q0 = s * kBatchMaxNV;
+    if (initial)
+        for (int l = 0; l < nv; l++) {
+            res0[q0 + l] = (double)rn[l];
+            tol[q0 + l] = fmax(abs_tol, rel_tol * sqrt(res0[q0 + l]));
+        }
+    for (int l = 0; l < nv; l++)
+        if (!S.status[q0 + l])
+            res[q0 + l] = (double)rn[l];
+    int flag = 1, bad = 0;
+    for (int l = 0; l < nv; l++) {
+        if (S.status[q0 + l])
+            continue;
+        const F ar = (F)sqrt(res[q0 + l]);
+        if (ar > (F)tol[q0 + l]) {
+            flag = 0;
+        } else if (ar == ar) {
+            S.status[q0 + l] = 1;
+        } else {
+            flag = 0;
+            bad = 1;
+        }
+    }
+    if (flag || bad)
+        return true;
+    if (iconv != nullptr)
+        for (int l = 0; l < nv; l++)
+            if (S.status[q0 + l])
+                iconv[q0 + l] = (F)0;
+    return false;
+}
+
+template <typename F>
+__device__ void bhalt(BState<F> S, int ns, int s) {
+    if (!S.sy[BY_HALT * ns + s]) {
+        S.sy[BY_HALT * ns + s] = 1;
+        atomicAdd(S.ndone, 1);
+    }
+}
+
+// Per-system sums of the tile partials (tile order) and the solver's scalar update,
+// one thread per system (pbicgstab.inl:108-204 scalar steps; ir.inl / residual
+// replacement for the FP32 recurrences).  F is the precision of the recurrences.
+template <typename F>
+__global__ void kb_finish(BLayout L, BState<F> S, BFinArgs f) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ns = L.nsys;
+    if (s >= ns)
+        return;
+    int *sy = S.sy;
+    auto SY = [&](int fld) -> int & { return sy[fld * ns + s]; };
+    const bool init = f.fin == FIN_INIT || f.fin == FIN_MX_INIT;
+    if (!init && SY(BY_HALT))
+        return;
+    if (f.gate != nullptr && !f.gate[s])
+        return;
+    const int nv = L.nv[s];
+    double sums[kBatchDots][kBatchMaxNV];
+    for (int k = 0; k < kBatchDots; k++)
+        for (int l = 0; l < kBatchMaxNV; l++)
+            sums[k][l] = 0.0;
+    for (int t = L.stile[s]; t < L.stile[s + 1]; t++) {
+        const double *p = S.part + (int64_t)t * PSTRIDE;
+        for (int k = 0; k < f.nd; k++)
+            for (int l = 0; l < nv; l++)
+                sums[k][l] += p[k * kBatchMaxNV + l];
+    }
+    const int NS4 = ns * kBatchMaxNV, q0 = s * kBatchMaxNV;
+    auto SC = [&](int fld, int l) -> F & { return S.sc[fld * NS4 + q0 + l]; };
+    F v[kBatchMaxNV];
+    switch (f.fin) {
+    case FIN_INIT: { // rho = (r, r0) = (r, r); initial check
+        for (int l = 0; l < nv; l++) {
+            v[l] = (F)sums[0][l];
+            SC(BS_RHO, l) = v[l];
+        }
+        if (bconv_check<F>(S, ns, s, nv, v, true, f.abs_tol, f.rel_tol, &S.sc[BS_ICONV * NS4]))
+            bhalt(S, ns, s);
+        break;
+    }
+    case FIN_ALPHA:
+        SY(BY_ITER)++;
+        for (int l = 0; l < nv; l++) {
+            const F al = SC(BS_RHO, l) / (F)sums[0][l];
+            SC(BS_ALPHA, l) = al;
+            SC(BS_NALPHA, l) = al * (F)-1.0;
+        }
+        SY(BY_RESTART) = 0;
+        SY(BY_RSTNOW) = 0;
+        break;
+    case FIN_OMEGA: {
+        for (int l = 0; l < nv; l++) {
+            const F s2 = (F)sums[0][l], s3 = (F)sums[1][l], s4 = (F)sums[2][l];
+            const F om = s2 / s3;
+            SC(BS_OMEGA, l) = om;
+            SC(BS_NOMEGA, l) = om * (F)-1.0;
+            const F m = SC(BS_ICONV, l);
+            SC(BS_AC, l) = m != (F)0 ? m * SC(BS_ALPHA, l) : (F)0;
+            SC(BS_OC, l) = m != (F)0 ? m * om : (F)0;
+            F res = om * s2;
+            res = (F)1.0 * s4 + (F)(-1.0) * res;
+            v[l] = res;
+        }
+        if (bconv_check<F>(S, ns, s, nv, v, false, f.abs_tol, f.rel_tol, &S.sc[BS_ICONV * NS4]))
+            SY(BY_DONE) = 1;
+        break;
+    }
+    case FIN_BETA:
+        for (int l = 0; l < nv; l++) {
+            const F s1 = (F)sums[0][l];
+            const F s2 = s1 / SC(BS_RHO, l);
+            const F s3 = SC(BS_ALPHA, l) / SC(BS_OMEGA, l);
+            const F be = s2 * s3;
+            SC(BS_BETA, l) = be;
+            SC(BS_RHO, l) = s1;
+            const F t = SC(BS_OMEGA, l) * be;
+            SC(BS_NBO, l) = t * (F)-1.0;
+            if (f.theta > 0 && !S.status[q0 + l] && !SY(BY_RESTART) &&
+                fabs(sums[0][l]) < f.theta * sqrt(sums[1][l] * sums[2][l]))
+                SY(BY_RSTNOW) = 1;
+        }
+        SY(BY_REPLACED) = 0;
+        if (SY(BY_DONE) || SY(BY_ITER) >= f.max_iters)
+            bhalt(S, ns, s);
+        break;
+    case FIN_MX_INIT: { // FP64 residual norms; FP32 rho; initial check on FP64 values
+        double rn[kBatchMaxNV];
+        double *maxres = S.cv + BC_MAXRES * NS4;
+        for (int l = 0; l < nv; l++) {
+            rn[l] = sums[0][l];
+            SC(BS_RHO, l) = (F)sums[0][l];
+            maxres[q0 + l] = sums[0][l];
+        }
+        SY(BY_REPLACE) = SY(BY_REPLACED) = SY(BY_RESTART) = SY(BY_RSTNOW) = 0;
+        BState<double> S64{nullptr, S.cv, S.status, S.sy, S.ndone, S.part, nullptr};
+        if (bconv_check<double>(S64, ns, s, nv, rn, true, f.abs_tol, f.rel_tol, nullptr)) {
+            bhalt(S, ns, s);
+        } else {
+            for (int l = 0; l < nv; l++)
+                if (S.status[q0 + l])
+                    SC(BS_ICONV, l) = (F)0;
+        }
+        break;
+    }
+    case FIN_MX_OMEGA: {
+        double *res = S.cv + BC_RES * NS4, *tol = S.cv + BC_TOL * NS4,
+               *maxres = S.cv + BC_MAXRES * NS4;
+        int req = 0, brk = 0;
+        for (int l = 0; l < nv; l++) {
+            const F s2 = (F)sums[0][l], s3 = (F)sums[1][l], s4 = (F)sums[2][l];
+            const F om = s2 / s3;
+            SC(BS_OMEGA, l) = om;
+            SC(BS_NOMEGA, l) = om * (F)-1.0;
+            const F m = SC(BS_ICONV, l);
+            SC(BS_AC, l) = m != (F)0 ? m * SC(BS_ALPHA, l) : (F)0;
+            SC(BS_OC, l) = m != (F)0 ? m * om : (F)0;
+            F r = om * s2;
+            r = (F)1.0 * s4 + (F)(-1.0) * r;
+            if (S.status[q0 + l])
+                continue;
+            if (!(isfinite(om) && om != (F)0 && isfinite(SC(BS_ALPHA, l)) && isfinite(r))) {
+                SC(BS_AC, l) = SC(BS_OC, l) = (F)0;
+                brk = 1;
+                continue;
+            }
+            const double rr = (double)r;
+            res[q0 + l] = rr;
+            if (rr <= tol[q0 + l] * tol[q0 + l] || rr <= f.delta2 * maxres[q0 + l])
+                req = 1;
+        }
+        if (brk) {
+            SY(BY_RESTART) = 1;
+            req = 1;
+        }
+        if (req)
+            SY(BY_REPLACE) = 1;
+        break;
+    }
+    case FIN_MX_REPL:   // FP64 true residual: convergence decided here
+    case FIN_MX_RST: {  // the same after a restart (r0 = p = r)
+        double *res = S.cv + BC_RES * NS4, *tol = S.cv + BC_TOL * NS4,
+               *maxres = S.cv + BC_MAXRES * NS4;
+        SY(BY_SUB)++;
+        int flag = 1, bad = 0, rst = 0;
+        for (int l = 0; l < nv; l++) {
+            if (f.fin == FIN_MX_RST)
+                SC(BS_RHO, l) = (F)sums[0][l];
+            if (S.status[q0 + l])
+                continue;
+            // recurrence 10x off the true residual, or no progress since the last
+            // check: restart the recurrence from the true residual
+            if (f.fin == FIN_MX_REPL &&
+                ((f.gap2 > 0 && sums[0][l] > f.gap2 * res[q0 + l]) || sums[0][l] > maxres[q0 + l]))
+                rst = 1;
+            res[q0 + l] = sums[0][l];
+            maxres[q0 + l] = sums[0][l];
+            if (sqrt(sums[0][l]) <= tol[q0 + l])
+                S.status[q0 + l] = 1;
+            else
+                flag = 0;
+            if (sums[0][l] != sums[0][l])
+                bad = 1;
+        }
+        if (flag || bad) {
+            SY(BY_DONE) = 1;
+            if (f.fin == FIN_MX_RST)
+                bhalt(S, ns, s);
+        } else {
+            for (int l = 0; l < nv; l++)
+                if (S.status[q0 + l])
+                    SC(BS_ICONV, l) = (F)0;
+        }
+        if (f.fin == FIN_MX_REPL) {
+            if (rst)
+                SY(BY_RESTART) = 1;
+            SY(BY_REPLACE) = 0;
+            SY(BY_REPLACED) = 1;
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// State reset: scalars 0, masks 1, relax factors, flags 0 (a kernel, so the reset can
+// sit next to captured work without memset nodes).
+template <typename F>
+__global__ void kb_state_init(BState<F> S, int ns, F relax) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NS4 = ns * kBatchMaxNV;
+    if (q < NS4) {
+        for (int f = 0; f < BS_NF; f++)
+            S.sc[f * NS4 + q] = (F)0;
+        S.sc[BS_ICONV * NS4 + q] = (F)1;
+        S.sc[BS_RELAX * NS4 + q] = relax;
+        for (int f = 0; f < BC_NF; f++)
+            S.cv[f * NS4 + q] = 0.0;
+        S.status[q] = 0;
+    }
+    if (q < ns)
+        for (int f = 0; f < BY_NF; f++)
+            S.sy[f * ns + q] = 0;
+    if (q == 0)
+        *S.ndone = 0;
+}
+
+// Per-system sums of the tile partials into out[(s*4 + l)*nd + k].
+__global__ void kb_collect(BLayout L, const double *part, int nd, double *out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= L.nsys)
+        return;
+    for (int k = 0; k < nd; k++)
+        for (int l = 0; l < kBatchMaxNV; l++) {
+            double v = 0;
+            for (int t = L.stile[s]; t < L.stile[s + 1]; t++)
+                v += part[(int64_t)t * PSTRIDE + k * kBatchMaxNV + l];
+            out[((int64_t)s * kBatchMaxNV + l) * nd + k] = v;
+        }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Batch (layout and matrix in HBM)
+// ---------------------------------------------------------------------------
+void Batch::build(int ns, const int *sys_rows, const int *nvs, const int *r, const int *c,
+                  const double *v) {
+    XAMG_REQUIRE(ns > 0, "batch: no systems");
+    nsys = ns;
+    h_srow.resize(ns + 1);
+    h_nv.assign(nvs, nvs + ns);
+    h_voff.resize(ns + 1);
+    XAMG_REQUIRE(sys_rows[0] == 0, "batch: system rows must start at 0");
+    h_voff[0] = 0;
+    for (int s = 0; s < ns; s++) {
+        XAMG_REQUIRE(sys_rows[s + 1] > sys_rows[s], "batch: every system needs at least one row");
+        XAMG_REQUIRE(nvs[s] >= 1 && nvs[s] <= kBatchMaxNV, "batch: nv per system must be 1..4");
+        h_srow[s] = (uint32_t)sys_rows[s];
+        h_voff[s + 1] = h_voff[s] + (int64_t)(sys_rows[s + 1] - sys_rows[s]) * nvs[s];
+    }
+    h_srow[ns] = (uint32_t)sys_rows[ns];
+    nrows = sys_rows[ns];
+    nelem = h_voff[ns];
+    nnz = r[nrows];
+    XAMG_REQUIRE(r[0] == 0, "batch: row pointers must start at 0");
+    hrow.resize(nrows + 1);
+    hcol.resize(nnz);
+    hval.assign(v, v + nnz);
+    for (int64_t i = 0; i <= nrows; i++) {
+        XAMG_REQUIRE(i == 0 || r[i] >= r[i - 1], "batch: row pointers must be non-decreasing");
+        hrow[i] = (uint32_t)r[i];
+    }
+    for (int s = 0; s < ns; s++)
+        for (int64_t i = sys_rows[s]; i < sys_rows[s + 1]; i++)
+            for (int j = r[i]; j < r[i + 1]; j++) {
+                XAMG_REQUIRE(c[j] >= sys_rows[s] && c[j] < sys_rows[s + 1],
+                             "batch: a column index leaves its system's block");
+                hcol[j] = (uint32_t)c[j];
+            }
+    auto up = [](DBuf &d, const void *p, size_t bytes) {
+        d.alloc(bytes);
+        if (bytes)
+            XAMG_HIP_CHECK(hipMemcpy(d.p, p, bytes, hipMemcpyHostToDevice));
+    };
+    up(srow, h_srow.data(), sizeof(uint32_t) * h_srow.size());
+    up(nv, h_nv.data(), sizeof(int) * h_nv.size());
+    up(voff, h_voff.data(), sizeof(int64_t) * h_voff.size());
+    up(row, hrow.data(), sizeof(uint32_t) * hrow.size());
+    up(col, hcol.data(), sizeof(uint32_t) * hcol.size());
+    up(val64, hval.data(), sizeof(double) * hval.size());
+    val32.release();
+    dinv64.release();
+    dinv32.release();
+    tile_rows = 0;
+    set_tiles(batch_cfg().tile_rows);
+}
+
+void Batch::set_tiles(int rows) {
+    XAMG_REQUIRE(rows >= 1, "batch: tile rows must be positive");
+    if (rows == tile_rows)
+        return;
+    tile_rows = rows;
+    std::vector<int> ts;
+    std::vector<uint32_t> tr;
+    h_stile.assign(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+        h_stile[s] = (int)ts.size();
+        for (uint32_t i = h_srow[s]; i < h_srow[s + 1]; i += (uint32_t)rows) {
+            ts.push_back(s);
+            tr.push_back(i);
+        }
+    }
+    h_stile[nsys] = (int)ts.size();
+    tr.push_back((uint32_t)nrows);
+    ntiles = (int)ts.size();
+    auto up = [](DBuf &d, const void *p, size_t bytes) {
+        d.alloc(bytes);
+        XAMG_HIP_CHECK(hipMemcpy(d.p, p, bytes, hipMemcpyHostToDevice));
+    };
+    up(tsys, ts.data(), sizeof(int) * ts.size());
+    up(trow, tr.data(), sizeof(uint32_t) * tr.size());
+    up(stile, h_stile.data(), sizeof(int) * h_stile.size());
+}
+
+BLayout Batch::layout() const {
+    BLayout L;
+    L.nsys = nsys;
+    L.ntiles = ntiles;
+    L.srow = srow.as<uint32_t>();
+    L.nv = nv.as<int>();
+    L.voff = voff.as<int64_t>();
+    L.tsys = tsys.as<int>();
+    L.trow = trow.as<uint32_t>();
+    L.stile = stile.as<int>();
+    L.row = row.as<uint32_t>();
+    L.col = col.as<uint32_t>();
+    return L;
+}
+
+const void *Batch::values(bool fp32) {
+    if (!fp32)
+        return val64.p;
+    if (!val32.p) {
+        std::vector<float> v(std::max<int64_t>(nnz, 1));
+        for (int64_t j = 0; j < nnz; j++)
+            v[j] = (float)hval[j];
+        val32.alloc(sizeof(float) * v.size());
+        XAMG_HIP_CHECK(hipMemcpy(val32.p, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
+    }
+    return val32.p;
+}
+
+// csr_matrix.h:323-355: 1.0 / a_ii computed in double from the values in use
+template <typename T>
+const T *Batch::inv_diag() {
+    DBuf &d = std::is_same<T, double>::value ? dinv64 : dinv32;
+    if (!d.p) {
+        std::vector<T> h(std::max<int64_t>(nrows, 1), (T)0);
+        for (int64_t i = 0; i < nrows; i++)
+            for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
+                if (hcol[j] == (uint32_t)i)
+                    h[i] = (T)(1.0 / (double)(T)hval[j]);
+        d.alloc(sizeof(T) * h.size());
+        XAMG_HIP_CHECK(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    }
+    return d.as<T>();
+}
+template const double *Batch::inv_diag<double>();
+template const float *Batch::inv_diag<float>();
+
+// ---------------------------------------------------------------------------
+// Batched (P)BiCGStab, FP64, or FP32 recurrences with FP64 residual replacement
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename V, typename T, int EPI, int ND>
+void launch_bspmv(const BLayout &L, const int *halt, const BSpmvArgs<T> &a, double *part,
+                  hipStream_t st) {
+    const int G = std::max(1, std::min(batch_cfg().grid, L.ntiles));
+    hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+}
+template <typename T, int OP, int ND>
+void launch_bew(const BLayout &L, const int *halt, const BEwArgs<T> &a, double *part,
+                hipStream_t st) {
+    const int G = std::max(1, std::min(batch_cfg().grid, L.ntiles));
+    hipLaunchKernelGGL((kb_ew<T, OP, ND>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+}
+template <typename F>
+void launch_bfin(const BLayout &L, const BState<F> &S, const BFinArgs &f, hipStream_t st) {
+    hipLaunchKernelGGL((kb_finish<F>), dim3((L.nsys + 63) / 64), dim3(64), 0, st, L, S, f);
+}
+
+template <typename T>
+class BatchKrylov : public BatchKrylovBase {
+    static constexpr bool kMixed = std::is_same<T, float>::value;
+
+  public:
+    Batch &B;
+    bool pre = false;
+    int sweeps = 2;
+    double relax = 0.99;
+    int max_iters = 50;
+    double abs_tol = 0, rel_tol = 0, delta2 = 1e-4, theta = 1e-7, gap2 = 100.0;
+    std::vector<DBuf> vec;
+    DBuf sc, cv, status, sy, ndone, part;
+    int *pinned = nullptr;
+    BState<T> S;
+    hipGraphExec_t gexec = nullptr;
+    const double *g_x = nullptr, *g_b = nullptr;
+    int g_k = 0, g_tiles = 0, g_grid = 0;
+
+    explicit BatchKrylov(Batch &b) : B(b) {}
+    ~BatchKrylov() override {
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
+        if (pinned)
+            (void)hipHostFree(pinned);
+    }
+    T *V(int k) { return vec[k].as<T>(); }
+
+    void setup(const GlobalParams &gp) {
+        const bool meta = gp.has("meta_solver");
+        const ParamList &sp = gp.get("solver");
+        const std::string m = sp.get_string("method");
+        XAMG_REQUIRE(m == "BiCGStab" || m == "PBiCGStab" || m == "MergedBiCGStab" ||
+                         m == "MergedPBiCGStab",
+                     "batch: the solver must be BiCGStab or PBiCGStab");
+        pre = (m == "PBiCGStab" || m == "MergedPBiCGStab");
+        if (pre) {
+            XAMG_REQUIRE(gp.has("preconditioner"), "batch: PBiCGStab needs a preconditioner block");
+            const ParamList &pp = gp.get("preconditioner");
+            XAMG_REQUIRE(pp.get_string("method") == "Jacobi",
+                         "batch: the preconditioner must be Jacobi");
+            sweeps = (int)pp.get_int("max_iters");
+            relax = pp.get_float("relax_factor");
+            XAMG_REQUIRE(sweeps >= 1, "batch: Jacobi needs at least one sweep");
+        }
+        const ParamList &top = meta ? gp.get("meta_solver") : sp;
+        abs_tol = top.get_float("abs_tolerance");
+        rel_tol = top.get_float("rel_tolerance");
+        max_iters = (int)sp.get_int("max_iters");
+        if (kMixed) {
+            const double step = top.has("replace_step") ? top.get_float("replace_step") : 1e-2;
+            delta2 = step * step;
+            theta = top.has("restart_threshold") ? top.get_float("restart_threshold") : 1e-7;
+            const double gap = top.has("restart_gap") ? top.get_float("restart_gap") : 10.0;
+            gap2 = gap * gap;
+        }
+        const int nb = pre ? 9 : 6;
+        vec.clear();
+        for (int i = 0; i < nb; i++)
+            vec.emplace_back(sizeof(T) * std::max<int64_t>(B.nelem, 1));
+        const int NS4 = B.nsys * kBatchMaxNV;
+        sc.alloc(sizeof(T) * BS_NF * NS4);
+        cv.alloc(sizeof(double) * BC_NF * NS4);
+        status.alloc(sizeof(int) * NS4);
+        sy.alloc(sizeof(int) * BY_NF * B.nsys);
+        ndone.alloc(256);
+        XAMG_HIP_CHECK(hipHostMalloc((void **)&pinned, 2 * sizeof(int), hipHostMallocDefault));
+        B.values(kMixed);
+        B.inv_diag<T>();
+        S.sc = sc.as<T>();
+        S.cv = cv.as<double>();
+        S.status = status.as<int>();
+        S.sy = sy.as<int>();
+        S.ndone = ndone.as<int>();
+    }
+
+    const int *halt() const { return S.sy + BY_HALT * B.nsys; }
+    const int *syf(int f) const { return S.sy + f * B.nsys; }
+    T *scf(int f) const { return S.sc + f * B.nsys * kBatchMaxNV; }
+
+    BFinArgs fin(int op, int nd, const int *gate = nullptr) const {
+        BFinArgs f;
+        f.fin = op;
+        f.nd = nd;
+        f.abs_tol = abs_tol;
+        f.rel_tol = rel_tol;
+        f.delta2 = delta2;
+        f.gap2 = gap2;
+        f.max_iters = max_iters;
+        f.gate = gate;
+        return f;
+    }
+
+    // z = M^{-1} q: Jacobi sweeps from zero (jacobi.inl no-check mode, out of place)
+    void jacobi(const T *q, T *out, T *tmp, hipStream_t st, const BLayout &L) {
+        const T *d = B.inv_diag<T>();
+        const T *src = nullptr;
+        {
+            T *dst = ((sweeps - 1) % 2 == 0) ? out : tmp;
+            BEwArgs<T> a;
+            a.z = dst;
+            a.x = q;
+            a.d = d;
+            a.ca = scf(BS_RELAX);
+            launch_bew<T, EW_JACOBI0, 0>(L, halt(), a, nullptr, st);
+            src = dst;
+        }
+        for (int k = 2; k <= sweeps; k++) {
+            T *dst = ((sweeps - k) % 2 == 0) ? out : tmp;
+            BSpmvArgs<T> a;
+            a.val = B.values(kMixed);
+            a.x = src;
+            a.xin = src;
+            a.y = dst;
+            a.b = q;
+            a.d = d;
+            a.wcoef = scf(BS_RELAX);
+            launch_bspmv<T, T, EPI_JACOBI, 0>(L, halt(), a, nullptr, st);
+            src = dst;
+        }
+    }
+
+    void init(double *x, const double *b, hipStream_t st, const BLayout &L) {
+        const int NS4 = B.nsys * kBatchMaxNV;
+        hipLaunchKernelGGL((kb_state_init<T>), dim3((NS4 + 255) / 256), dim3(256), 0, st, S,
+                           B.nsys, (T)relax);
+        T *r0 = V(0), *r = V(1), *p = V(2);
+        if constexpr (kMixed) {
+            BSpmvArgs<double> a;
+            a.val = B.values(false);
+            a.x = x;
+            a.b = b;
+            a.yo = r0;
+            a.yo2 = r;
+            a.yo3 = p;
+            launch_bspmv<double, double, EPI_RESID_MX, 1>(L, nullptr, a, part.as<double>(), st);
+            launch_bfin<T>(L, S, fin(FIN_MX_INIT, 1), st);
+        } else {
+            BSpmvArgs<double> a;
+            a.val = B.values(false);
+            a.x = x;
+            a.b = b;
+            a.y = r0;
+            a.y2 = r;
+            a.y3 = p;
+            launch_bspmv<double, double, EPI_RESID, 1>(L, nullptr, a, part.as<double>(), st);
+            launch_bfin<T>(L, S, fin(FIN_INIT, 1), st);
+        }
+    }
+
+    void iteration(double *x, const double *b, hipStream_t st, const BLayout &L) {
+        T *r0 = V(0), *r = V(1), *p = V(2), *v = V(3), *s = V(4), *t = V(5);
+        T *p_ = pre ? V(6) : p, *s_ = pre ? V(7) : s, *jt = pre ? V(8) : nullptr;
+        double *pt = part.as<double>();
+        const void *val = B.values(kMixed);
+        if (pre)
+            jacobi(p, p_, jt, st, L);
+        { // v = A p_, (v, r0) -> alpha
+            BSpmvArgs<T> a;
+            a.val = val;
+            a.x = p_;
+            a.y = v;
+            a.w = r0;
+            launch_bspmv<T, T, EPI_AXY, 1>(L, halt(), a, pt, st);
+            launch_bfin<T>(L, S, fin(FIN_ALPHA, 1), st);
+        }
+        { // s = r - alpha v
+            BEwArgs<T> a;
+            a.z = s;
+            a.x = r;
+            a.y = v;
+            a.cb = scf(BS_NALPHA);
+            launch_bew<T, EW_AXPBY_Z, 0>(L, halt(), a, nullptr, st);
+        }
+        if (pre)
+            jacobi(s, s_, jt, st, L);
+        { // t = A s_, (t, s), (t, t), (s, s) -> omega, residual, check
+            BSpmvArgs<T> a;
+            a.val = val;
+            a.x = s_;
+            a.y = t;
+            a.w = s;
+            launch_bspmv<T, T, EPI_AXY, 3>(L, halt(), a, pt, st);
+            launch_bfin<T>(L, S, fin(kMixed ? FIN_MX_OMEGA : FIN_OMEGA, 3), st);
+        }
+        if constexpr (kMixed) {
+            { // x(FP64) += alpha_c p_ + omega_c s_
+                BEwArgs<double> a;
+                a.z = x;
+                a.xo = p_;
+                a.yo = s_;
+                a.ao = scf(BS_AC);
+                a.bo = scf(BS_OC);
+                a.freeze_zero = 1;
+                launch_bew<double, EW_MX_XUPD, 0>(L, halt(), a, nullptr, st);
+            }
+            { // FP64 residual replacement for the systems that asked for it
+                BSpmvArgs<double> a;
+                a.val = B.values(false);
+                a.x = x;
+                a.b = b;
+                a.yo = r;
+                a.gate = syf(BY_REPLACE);
+                launch_bspmv<double, double, EPI_RESID_MX, 1>(L, halt(), a, pt, st);
+                launch_bfin<T>(L, S, fin(FIN_MX_REPL, 1, syf(BY_REPLACE)), st);
+            }
+        } else {
+            BEwArgs<T> a;
+            a.z = (T *)x;
+            a.x = p_;
+            a.ca = scf(BS_AC);
+            a.y = s_;
+            a.cb = scf(BS_OC);
+            a.freeze_zero = 1;
+            launch_bew<T, EW_AXPBYPCZ, 0>(L, halt(), a, nullptr, st);
+        }
+        { // r = s - omega t, (r, r0) -> beta
+            BEwArgs<T> a;
+            a.z = r;
+            a.x = s;
+            a.y = t;
+            a.cb = scf(BS_NOMEGA);
+            a.w = r0;
+            if constexpr (kMixed) {
+                a.keep = syf(BY_REPLACED);
+                a.y2 = r0; // restart: r0 = r, rho = (r, r)
+                a.copy = syf(BY_RESTART);
+                launch_bew<T, EW_AXPBY_Z, 3>(L, halt(), a, pt, st);
+                BFinArgs fb = fin(FIN_BETA, 3);
+                fb.theta = theta;
+                launch_bfin<T>(L, S, fb, st);
+            } else {
+                launch_bew<T, EW_AXPBY_Z, 1>(L, halt(), a, pt, st);
+                launch_bfin<T>(L, S, fin(FIN_BETA, 1), st);
+            }
+        }
+        if constexpr (kMixed) { // (r, r0) lost: FP64 residual, r0 = p = r
+            BSpmvArgs<double> a;
+            a.val = B.values(false);
+            a.x = x;
+            a.b = b;
+            a.yo = r;
+            a.yo2 = r0;
+            a.yo3 = p;
+            a.gate = syf(BY_RSTNOW);
+            launch_bspmv<double, double, EPI_RESID_MX, 1>(L, halt(), a, pt, st);
+            launch_bfin<T>(L, S, fin(FIN_MX_RST, 1, syf(BY_RSTNOW)), st);
+        }
+        { // p = r + beta p - beta omega v
+            BEwArgs<T> a;
+            a.z = p;
+            a.x = r;
+            a.y = v;
+            a.cb = scf(BS_NBO);
+            a.cc = scf(BS_BETA);
+            a.copy = kMixed ? syf(BY_RESTART) : nullptr; // p = r on a restart
+            a.keep = kMixed ? syf(BY_RSTNOW) : nullptr;  // p = r already
+            launch_bew<T, EW_AXPBYPCZ, 0>(L, halt(), a, nullptr, st);
+        }
+    }
+
+    void solve(double *x, const double *b) override {
+        Runtime &rt = Runtime::get();
+        hipStream_t st = rt.stream;
+        const BatchCfg &cfg = batch_cfg();
+        B.set_tiles(cfg.tile_rows);
+        if (part.bytes < sizeof(double) * PSTRIDE * (size_t)B.ntiles)
+            part.alloc(sizeof(double) * PSTRIDE * (size_t)B.ntiles);
+        S.part = part.as<double>();
+        const BLayout L = B.layout();
+        const int K = std::max(1, cfg.batch);
+        hipEvent_t e0, e1;
+        XAMG_HIP_CHECK(hipEventCreate(&e0));
+        XAMG_HIP_CHECK(hipEventCreate(&e1));
+        XAMG_HIP_CHECK(hipEventRecord(e0, st));
+        init(x, b, st, L);
+        // one graph of K iterations, rebuilt when the vectors or the launch shape change
+        if (cfg.use_graph &&
+            (!gexec || g_x != x || g_b != b || g_k != K || g_tiles != B.ntiles || g_grid != cfg.grid)) {
+            if (gexec) {
+                (void)hipGraphExecDestroy(gexec);
+                gexec = nullptr;
+            }
+            hipGraph_t g;
+            XAMG_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; k++)
+                iteration(x, b, st, L);
+            XAMG_HIP_CHECK(hipStreamEndCapture(st, &g));
+            XAMG_HIP_CHECK(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            g_x = x;
+            g_b = b;
+            g_k = K;
+            g_tiles = B.ntiles;
+            g_grid = cfg.grid;
+        }
+        hipEvent_t ev[2];
+        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        // batches of K iterations; the counter of finished systems is read back after
+        // each batch, one batch behind (the next batch is queued before waiting)
+        const int nbatch = (max_iters + K - 1) / K;
+        int issued = 0, waited = 0;
+        bool finished = false;
+        auto enqueue = [&]() {
+            if (cfg.use_graph)
+                XAMG_HIP_CHECK(hipGraphLaunch(gexec, st));
+            else
+                for (int k = 0; k < K; k++)
+                    iteration(x, b, st, L);
+            XAMG_HIP_CHECK(hipMemcpyAsync(&pinned[issued % 2], S.ndone, sizeof(int),
+                                          hipMemcpyDeviceToHost, st));
+            XAMG_HIP_CHECK(hipEventRecord(ev[issued % 2], st));
+            issued++;
+        };
+        while (!finished && waited < nbatch) {
+            while (issued < nbatch && issued < waited + 2)
+                enqueue();
+            XAMG_HIP_CHECK(hipEventSynchronize(ev[waited % 2]));
+            if (pinned[waited % 2] >= B.nsys)
+                finished = true;
+            waited++;
+        }
+        XAMG_HIP_CHECK(hipEventRecord(e1, st));
+        XAMG_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        B.last_ms = ms;
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+
+    void info(int *iters, int *sub_iters, int *converged) override {
+        std::vector<int> h(BY_NF * B.nsys), stt(B.nsys * kBatchMaxNV);
+        XAMG_HIP_CHECK(hipMemcpy(h.data(), S.sy, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+        XAMG_HIP_CHECK(hipMemcpy(stt.data(), S.status, sizeof(int) * stt.size(), hipMemcpyDeviceToHost));
+        int q = 0;
+        for (int s = 0; s < B.nsys; s++) {
+            if (iters)
+                iters[s] = h[BY_ITER * B.nsys + s];
+            if (sub_iters)
+                sub_iters[s] = h[BY_SUB * B.nsys + s];
+            if (converged)
+                for (int l = 0; l < B.h_nv[s]; l++)
+                    converged[q++] = stt[s * kBatchMaxNV + l];
+        }
+    }
+};
+
+} // namespace
+
+std::unique_ptr<BatchKrylovBase> make_batch_solver(Batch &B, const GlobalParams &gp0) {
+    GlobalParams gp = gp0;
+    gp.set_defaults();
+    XAMG_REQUIRE(gp.has("solver"), "batch: the parameters need a solver block");
+    bool mixed = false;
+    if (gp.has("meta_solver")) {
+        const ParamList &m = gp.get("meta_solver");
+        XAMG_REQUIRE(m.get_string("method") == "IterativeRefinement",
+                     "batch: the meta solver must be IterativeRefinement");
+        mixed = m.has("subsolver_reduced_precision") && m.get_bool("subsolver_reduced_precision");
+        XAMG_REQUIRE(mixed, "batch: IterativeRefinement runs with subsolver_reduced_precision");
+    }
+    if (mixed) {
+        auto s = std::make_unique<BatchKrylov<float>>(B);
+        s->setup(gp);
+        return s;
+    }
+    auto s = std::make_unique<BatchKrylov<double>>(B);
+    s->setup(gp);
+    return s;
+}
+
+void batch_residuals(Batch &B, const double *x, const double *b, double *relres) {
+    hipStream_t st = Runtime::get().stream;
+    B.set_tiles(batch_cfg().tile_rows);
+    const BLayout L = B.layout();
+    DBuf y(sizeof(double) * std::max<int64_t>(B.nelem, 1));
+    DBuf part(sizeof(double) * PSTRIDE * (size_t)B.ntiles);
+    DBuf sums(sizeof(double) * 2 * B.nsys * kBatchMaxNV);
+    {
+        BSpmvArgs<double> a;
+        a.val = B.values(false);
+        a.x = x;
+        a.b = b;
+        a.y = y.as<double>();
+        launch_bspmv<double, double, EPI_RESID, 1>(L, nullptr, a, part.as<double>(), st);
+        hipLaunchKernelGGL(kb_collect, dim3((B.nsys + 63) / 64), dim3(64), 0, st, L,
+                           part.as<const double>(), 1, sums.as<double>());
+    }
+    {
+        BEwArgs<double> a;
+        a.z = y.as<double>();
+        a.x = b;
+        launch_bew<double, EW_COPY3, 1>(L, nullptr, a, part.as<double>(), st);
+        hipLaunchKernelGGL(kb_collect, dim3((B.nsys + 63) / 64), dim3(64), 0, st, L,
+                           part.as<const double>(), 1, sums.as<double>() + B.nsys * kBatchMaxNV);
+    }
+    std::vector<double> h(2 * B.nsys * kBatchMaxNV);
+    XAMG_HIP_CHECK(hipMemcpyAsync(h.data(), sums.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    XAMG_HIP_CHECK(hipStreamSynchronize(st));
+    int q = 0;
+    for (int s = 0; s < B.nsys; s++)
+        for (int l = 0; l < B.h_nv[s]; l++) {
+            const double rr = h[s * kBatchMaxNV + l], bb = h[B.nsys * kBatchMaxNV + s * kBatchMaxNV + l];
+            relres[q++] = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+        }
+}
+
+} // namespace xamg

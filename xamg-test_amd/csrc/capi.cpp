@@ -6,6 +6,7 @@
 
 #include "params.hpp"
 #include "runtime.hpp"
+#include "batch.hpp"
 #include "solvers.hpp"
 
 #include <sys/time.h>
@@ -25,6 +26,10 @@ void aniso27(int nx, int ny, int nz, double kx, double ky, double kz, uint32_t s
              int *col, double *val);
 int64_t var5_nnz(int s);
 void var5(int s, uint32_t seed, int *row, int *col, double *val);
+void batch_var5_layout(int nsys, uint32_t seed, int smin, int smax, int maxnv, int *side, int *nv);
+int64_t batch_var5_nnz(int nsys, const int *side);
+void batch_var5(int nsys, const int *side, uint32_t seed, int *sys_rows, int *row, int *col,
+                double *val);
 } // namespace xamg
 
 using namespace xamg;
@@ -77,6 +82,7 @@ std::vector<HostCsr *> g_csr;
 std::vector<Mat *> g_mats;
 std::vector<Sol *> g_sols;
 std::vector<amg::Hierarchy *> g_hiers;
+std::vector<Batch *> g_batches;
 
 // MultiGrid keys of a parameter dictionary: the first list whose method is MultiGrid,
 // with the reference defaults applied (hypre_base.h:124-215 reads the same keys).
@@ -251,6 +257,14 @@ int XAMG_finalize() {
         for (auto &m : g_mats) {
             delete m;
             m = nullptr;
+        }
+        for (auto &h : g_hiers) {
+            delete h;
+            h = nullptr;
+        }
+        for (auto &b : g_batches) {
+            delete b;
+            b = nullptr;
         }
     });
 }
@@ -1031,6 +1045,98 @@ int xamgx_hierarchy_destroy(XAMG_hierarchy_id hid) {
         if (hid < g_hiers.size()) {
             delete g_hiers[hid];
             g_hiers[hid] = nullptr;
+        }
+    });
+}
+
+} // extern "C"
+
+// ---- batches of independent systems ----
+extern "C" {
+
+int xamgx_gen_batch_var5_layout(int nsys, uint32_t seed, int smin, int smax, int maxnv, int *side,
+                                int *nv) {
+    return guard([&] {
+        XAMG_REQUIRE(nsys > 0 && smin >= 2 && smax >= smin && maxnv >= 1 && maxnv <= 4,
+                     "invalid batch layout request");
+        batch_var5_layout(nsys, seed, smin, smax, maxnv, side, nv);
+    });
+}
+int64_t xamgx_gen_batch_var5_nnz(int nsys, const int *side) { return batch_var5_nnz(nsys, side); }
+int xamgx_gen_batch_var5(int nsys, const int *side, uint32_t seed, int *sys_rows, int *row, int *col,
+                         double *val) {
+    return guard([&] { batch_var5(nsys, side, seed, sys_rows, row, col, val); });
+}
+
+int xamgx_batch_create(int nsys, const int *sys_rows, const int *nv, const int *row, const int *col,
+                       const double *val, XAMG_batch_id *bid) {
+    return guard([&] {
+        auto *B = new Batch();
+        try {
+            B->build(nsys, sys_rows, nv, row, col, val);
+        } catch (...) {
+            delete B;
+            throw;
+        }
+        g_batches.push_back(B);
+        *bid = g_batches.size() - 1;
+    });
+}
+int xamgx_batch_setup(XAMG_batch_id bid, XAMG_param_id pid) {
+    return guard([&] {
+        Batch *B = slot(g_batches, bid, "batch");
+        GlobalParams *p = slot(g_params, pid, "param");
+        B->solver = make_batch_solver(*B, *p);
+    });
+}
+static double *batch_vec(Batch *B, XAMG_vector_id vid) {
+    Vec *v = slot(g_vecs, vid, "vector");
+    XAMG_REQUIRE(v->on_device(), "batch vectors must be device vectors");
+    XAMG_REQUIRE(v->size * v->nv == B->nelem,
+                 "batch vector size differs from the sum of rows x nv over the systems");
+    return v->dev.as<double>();
+}
+int xamgx_batch_solve(XAMG_batch_id bid, XAMG_vector_id x, XAMG_vector_id b) {
+    return guard([&] {
+        Batch *B = slot(g_batches, bid, "batch");
+        XAMG_REQUIRE(B->solver != nullptr, "batch: call xamgx_batch_setup first");
+        B->solver->solve(batch_vec(B, x), batch_vec(B, b));
+    });
+}
+int xamgx_batch_get_info(XAMG_batch_id bid, int *iters, int *checks, int *converged) {
+    return guard([&] {
+        Batch *B = slot(g_batches, bid, "batch");
+        XAMG_REQUIRE(B->solver != nullptr, "batch: call xamgx_batch_setup first");
+        B->solver->info(iters, checks, converged);
+    });
+}
+int xamgx_batch_residuals(XAMG_batch_id bid, XAMG_vector_id x, XAMG_vector_id b, double *relres) {
+    return guard([&] {
+        Batch *B = slot(g_batches, bid, "batch");
+        batch_residuals(*B, batch_vec(B, x), batch_vec(B, b), relres);
+    });
+}
+int xamgx_batch_last_ms(XAMG_batch_id bid, double *ms) {
+    return guard([&] { *ms = slot(g_batches, bid, "batch")->last_ms; });
+}
+int xamgx_batch_set_launch(int grid, int tile_rows, int batch, int use_graph) {
+    return guard([&] {
+        BatchCfg &c = batch_cfg();
+        if (grid > 0)
+            c.grid = grid;
+        if (tile_rows > 0)
+            c.tile_rows = tile_rows;
+        if (batch > 0)
+            c.batch = batch;
+        if (use_graph >= 0)
+            c.use_graph = use_graph;
+    });
+}
+int xamgx_batch_destroy(XAMG_batch_id bid) {
+    return guard([&] {
+        if (bid < g_batches.size()) {
+            delete g_batches[bid];
+            g_batches[bid] = nullptr;
         }
     });
 }

@@ -59,6 +59,7 @@ struct ConvState {
     int replace;            // set when the FP64 true residual must be recomputed
     int replaced;           // set by that recomputation for the following r update
     int restart;            // FP32 recurrence broke down: restart it from the FP64 residual
+    int rstnow;             // (r, r0) lost: restart at the end of this iteration
     double maxres[kMaxNV];  // squared residual at the last FP64 check
 };
 

@@ -1,4 +1,5 @@
 // Workload generators (host): the synthetic systems the solve path is measured on.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -166,6 +167,48 @@ void var5(int s, uint32_t seed, int *row, int *col, double *val) {
             val[dpos] = diag;
             row[r + 1] = (int)c;
         }
+}
+
+// Batch of independent 2D 5-point systems.  System k has side
+// round(smin * (smax/smin)^U_k) and nv_k = 1 + floor(maxnv * V_k) right-hand sides, with
+// U_k, V_k the pseudo_rand values at seed*2^27 + 2k and + 2k + 1; its coefficients are
+// var5(side_k, seed + 7919 (k + 1)).
+void batch_var5_layout(int nsys, uint32_t seed, int smin, int smax, int maxnv, int *side, int *nv) {
+    for (int k = 0; k < nsys; k++) {
+        const uint32_t i = (uint32_t)((uint64_t)seed * (1ull << 27) + 2ull * (uint64_t)k);
+        const double u = pseudo_rand_d(i), v = pseudo_rand_d(i + 1);
+        side[k] = (int)std::lround(smin * std::pow((double)smax / smin, u));
+        nv[k] = 1 + std::min(maxnv - 1, (int)(maxnv * v));
+    }
+}
+
+int64_t batch_var5_nnz(int nsys, const int *side) {
+    int64_t z = 0;
+    for (int k = 0; k < nsys; k++)
+        z += var5_nnz(side[k]);
+    return z;
+}
+
+// Concatenated block-diagonal CSR with global indices; sys_rows[k] is the first row of
+// system k (nsys + 1 entries).
+void batch_var5(int nsys, const int *side, uint32_t seed, int *sys_rows, int *row, int *col,
+                double *val) {
+    int64_t r0 = 0, z0 = 0;
+    row[0] = 0;
+    for (int k = 0; k < nsys; k++) {
+        const int s = side[k];
+        const int64_t n = (int64_t)s * s, z = var5_nnz(s);
+        sys_rows[k] = (int)r0;
+        std::vector<int> lr(n + 1), lc(z);
+        var5(s, seed + 7919u * (uint32_t)(k + 1), lr.data(), lc.data(), val + z0);
+        for (int64_t i = 1; i <= n; i++)
+            row[r0 + i] = (int)(z0 + lr[i]);
+        for (int64_t j = 0; j < z; j++)
+            col[z0 + j] = (int)(r0 + lc[j]);
+        r0 += n;
+        z0 += z;
+    }
+    sys_rows[nsys] = (int)r0;
 }
 
 } // namespace xamg

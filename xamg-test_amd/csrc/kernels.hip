@@ -144,6 +144,9 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
             ks->rho[l] = s1;
             F t = ks->omega[l] * be;
             ks->nbo[l] = t * (F)-1.0;
+            if (f.theta > 0 && cs != nullptr && !cs->status[l] && !cs->restart &&
+                fabs(sums[l]) < f.theta * sqrt(sums[nv + l] * sums[2 * nv + l]))
+                cs->rstnow = 1;
         }
         break;
     case FIN_NORM_INIT:
@@ -172,7 +175,7 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
             k32->rho[l] = (float)sums[l];
             cs->maxres[l] = sums[l];
         }
-        cs->replace = cs->replaced = cs->restart = 0;
+        cs->replace = cs->replaced = cs->restart = cs->rstnow = 0;
         conv_check(f, v, nv, 1);
         cs->halt = cs->done;
         if (!cs->done)
@@ -220,13 +223,22 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
             cs->replace = 1;
         break;
     }
-    case FIN_MX_REPL: {
+    case FIN_MX_REPL:
+    case FIN_MX_RST: {
         auto *k32 = (KrylovScalars<float> *)f.aux;
         cs->sub_iter++;
         int flag = 1, bad = 0;
+        int rst = 0;
         for (int l = 0; l < nv; l++) {
+            if (f.fin == FIN_MX_RST)
+                k32->rho[l] = (float)sums[l]; // r0 = r
             if (cs->status[l])
                 continue;
+            // the recurrence has lost track of the true residual (10x in norm) or the
+            // true residual did not fall since the last check: restart the recurrence
+            if (f.fin == FIN_MX_REPL &&
+                ((f.gap2 > 0 && sums[l] > f.gap2 * cs->res[l]) || sums[l] > cs->maxres[l]))
+                rst = 1;
             cs->res[l] = sums[l];
             cs->maxres[l] = sums[l];
             if (sqrt(sums[l]) <= cs->tol[l])
@@ -236,14 +248,21 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
             if (sums[l] != sums[l])
                 bad = 1;
         }
-        if (flag || bad)
+        if (flag || bad) {
             cs->done = 1;
-        else
+            if (f.fin == FIN_MX_RST)
+                cs->halt = 1;
+        } else {
             for (int l = 0; l < nv; l++)
                 if (cs->status[l])
                     k32->iconv[l] = 0.0f;
-        cs->replace = 0;
-        cs->replaced = 1;
+        }
+        if (f.fin == FIN_MX_REPL) {
+            if (rst)
+                cs->restart = 1;
+            cs->replace = 0;
+            cs->replaced = 1;
+        }
         break;
     }
     default:
@@ -251,6 +270,8 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
     }
     if (f.clear_on_fin != nullptr)
         *f.clear_on_fin = 0;
+    if (f.clear_on_fin2 != nullptr)
+        *f.clear_on_fin2 = 0;
 }
 
 // Block reduction of per-RHS accumulators, store of the block partial, and the
@@ -515,12 +536,10 @@ __global__ void __launch_bounds__(kBlock)
                 for (int v = 0; v < VPT; v++)
                     o32[v] = (float)out[v];
                 st_vec<float, VPT>(a.yo + e, o32);
-                if (a.copy_if == nullptr || *a.copy_if) {
-                    if (a.yo2 != nullptr)
-                        st_vec<float, VPT>(a.yo2 + e, o32);
-                    if (a.yo3 != nullptr)
-                        st_vec<float, VPT>(a.yo3 + e, o32);
-                }
+                if (a.yo2 != nullptr)
+                    st_vec<float, VPT>(a.yo2 + e, o32);
+                if (a.yo3 != nullptr)
+                    st_vec<float, VPT>(a.yo3 + e, o32);
             } else {
                 st_vec<T, VPT>(a.y + e, out);
                 if (a.y2 != nullptr)
@@ -573,6 +592,7 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
     if (r.gate != nullptr && !*r.gate)
         return;
     const bool keep = a.keep_if != nullptr && *a.keep_if;
+    const bool copy = a.copy_if != nullptr && *a.copy_if;
     const int tid = threadIdx.x;
     const int l = tid % NV;
     const int G = gridDim.x;
@@ -597,8 +617,13 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
             out = a.c0;
         } else if constexpr (OP == EW_AXPBY_Z) {
             out = keep ? a.z[e] : ca * a.x[e] + cb * a.y[e];
+            if (copy && a.y2 != nullptr)
+                a.y2[e] = out;
         } else if constexpr (OP == EW_AXPBYPCZ) {
-            out = (frozen || keep) ? a.z[e] : ca * a.x[e] + cb * a.y[e] + cc * a.z[e];
+            if (copy)
+                out = a.x[e];
+            else
+                out = (frozen || keep) ? a.z[e] : ca * a.x[e] + cb * a.y[e] + cc * a.z[e];
         } else if constexpr (OP == EW_JACOBI0) {
             const int64_t i = e / NV;
             out = (ca * a.d[i]) * a.x[e];
@@ -633,8 +658,13 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
         }
         a.z[e] = out;
         if constexpr (ND == 1) {
-            const double wv = a.w != nullptr ? (double)a.w[e] : (double)out;
+            const double wv = (a.w != nullptr && !copy) ? (double)a.w[e] : (double)out;
             dacc[0][0] += (double)out * wv;
+        } else if constexpr (ND == 3) {
+            const double o = (double)out, wv = copy ? o : (double)a.w[e];
+            dacc[0][0] += o * wv;
+            dacc[1][0] += o * o;
+            dacc[2][0] += wv * wv;
         }
     }
     if constexpr (ND > 0)
@@ -853,6 +883,7 @@ static void ew_nv(int op, int nd, const EwArgs<T> &a, const RedArgs &r, const Fi
     XAMG_EW_CASE(EW_FILL, 0)
     XAMG_EW_CASE(EW_AXPBY_Z, 0)
     XAMG_EW_CASE(EW_AXPBY_Z, 1)
+    XAMG_EW_CASE(EW_AXPBY_Z, 3)
     XAMG_EW_CASE(EW_AXPBYPCZ, 0)
     XAMG_EW_CASE(EW_JACOBI0, 0)
     XAMG_EW_CASE(EW_CHEB0, 0)

@@ -57,7 +57,9 @@ enum Fin : int {
     FIN_MX_INIT = 8,  // FP64 ||r0||^2: res0, tolerance, initial check; FP32 rho (aux)
     FIN_MX_OMEGA = 9, // as FIN_OMEGA, but a recurrence residual below the tolerance or
                       // below delta^2 * (last FP64 residual) requests an FP64 check
-    FIN_MX_REPL = 10  // FP64 true residual: per-RHS convergence, masks (aux), restart ref
+    FIN_MX_REPL = 10, // FP64 true residual: per-RHS convergence, masks (aux), restart ref
+    FIN_MX_RST = 11   // FP64 true residual of a restart (r0 = p = r): as FIN_MX_REPL, and
+                      // rho = (r, r); halts at once when every RHS has converged
 };
 
 template <typename F>
@@ -71,7 +73,13 @@ struct FinArgs {
     F *iconv = nullptr; // mask updated on partial convergence (nullptr: no mask)
     void *aux = nullptr;         // FIN_MX_*: the FP32 KrylovScalars of the iteration
     double delta2 = 0;           // FIN_MX_OMEGA: replacement step (squared)
+    double gap2 = 0;             // FIN_MX_REPL: restart when true > gap2 * recurrence residual^2
+                                 // (0: only on breakdown or no progress)
     int *clear_on_fin = nullptr; // flag reset to 0 by the final op
+    int *clear_on_fin2 = nullptr;
+    // FIN_BETA with the dots (r, r0), (r, r), (r0, r0): ask for a restart of the
+    // recurrence when |(r, r0)| < theta ||r|| ||r0|| (bi-orthogonality lost)
+    double theta = 0;
 };
 
 struct RedArgs {
@@ -103,7 +111,6 @@ struct SpmvArgs {
     T *y2 = nullptr, *y3 = nullptr; // extra copies of y (initial residual: r, p)
     // EPI_RESID_MX: FP32 copies of the FP64 residual (y is not written)
     float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr;
-    const int *copy_if = nullptr; // EPI_RESID_MX: yo2 / yo3 written only while *copy_if != 0
     // launch configuration chosen by the autotuner (0 / -1: global defaults)
     int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1;
 };
@@ -122,6 +129,9 @@ struct EwArgs {
     const void *yo = nullptr;                         // second other-precision input
     const float *ao = nullptr, *bo = nullptr;         // FP32 coefficients (EW_MX_XUPD)
     const int *keep_if = nullptr; // AXPBY_Z / AXPBYPCZ: leave z unchanged while *keep_if != 0
+    // while *copy_if != 0: AXPBY_Z also stores z into y2 and dots z with itself;
+    // AXPBYPCZ sets z = x (restart of a Krylov recurrence from its residual)
+    const int *copy_if = nullptr;
     // AXPBYPCZ / MX_XUPD: leave z unchanged for an RHS whose coefficients a and b are
     // both exactly 0 (the masked solution update of a converged RHS keeps its solution
     // even when that RHS's directions have broken down to NaN)

@@ -714,6 +714,8 @@ class ContinuedRefine : public Solver<double> {
     DBuf st32;
     KrylovScalars<float> *k32 = nullptr;
     double delta2 = 1e-4;
+    double theta = 1e-7;
+    double gap2 = 100.0;
     int max_inner = 1000;
 
     float *v32(int k) { return b32[k].as<float>(); }
@@ -730,6 +732,9 @@ class ContinuedRefine : public Solver<double> {
         max_inner = sp.has("max_iters") ? (int)sp.get_int("max_iters") : 1000;
         double step = prm.has("replace_step") ? prm.get_float("replace_step") : 1e-2;
         delta2 = step * step;
+        theta = prm.has("restart_threshold") ? prm.get_float("restart_threshold") : 1e-7;
+        const double gap = prm.has("restart_gap") ? prm.get_float("restart_gap") : 10.0;
+        gap2 = gap * gap;
         if (!conv_check)
             conv_check = 1; // convergence is always decided on the FP64 residual
         hist_cap = std::max(2, max_inner + 2);
@@ -773,6 +778,7 @@ class ContinuedRefine : public Solver<double> {
         a.abs_tol = abs_tol;
         a.rel_tol = rel_tol;
         a.aux = k32;
+        a.gap2 = gap2;
         a.hist = (f == FIN_MX_INIT) ? histbuf.as<double>() : nullptr; // initial row
         return a;
     }
@@ -803,6 +809,7 @@ class ContinuedRefine : public Solver<double> {
                 a.w = r0;
                 FinArgs<float> fa = fin32(FIN_ALPHA);
                 fa.clear_on_fin = &cs->restart;
+                fa.clear_on_fin2 = &cs->rstnow;
                 spmv<float>(*A, Prec::F32, EPI_AXY, 1, nv, a, red_args(H), fa);
             }
             {
@@ -834,15 +841,11 @@ class ContinuedRefine : public Solver<double> {
                 a.freeze_zero = 1;
                 ew<double>(EW_MX_XUPD, nv, a, H);
             }
-            { // FP64 residual replacement when requested; after a breakdown it also
-              // restarts the recurrence (r0 = p = r)
+            { // FP64 residual replacement when requested
                 SpmvArgs<double> a{};
                 a.x = x;
                 a.b = b;
                 a.yo = r;
-                a.yo2 = r0;
-                a.yo3 = p;
-                a.copy_if = &cs->restart;
                 RedArgs ra = red_args(H);
                 ra.gate = &cs->replace;
                 spmv<double>(*A, Prec::F64, EPI_RESID_MX, 1, nv, a, ra, fin64(FIN_MX_REPL));
@@ -856,14 +859,28 @@ class ContinuedRefine : public Solver<double> {
                 a.b = k32->nomega;
                 a.w = r0;
                 a.keep_if = &cs->replaced;
+                a.y2 = r0; // restart: r0 = r, rho = (r, r)
+                a.copy_if = &cs->restart;
                 RedArgs ra = red_args(H);
                 ra.set_halt_if_done = 1;
                 ra.cs = cs;
                 FinArgs<float> fa = fin32(FIN_BETA);
                 fa.clear_on_fin = &cs->replaced;
-                ew<float>(EW_AXPBY_Z, 1, nv, a, ra, fa);
+                fa.theta = theta;
+                ew<float>(EW_AXPBY_Z, 3, nv, a, ra, fa);
             }
-            { // p = r + beta p - beta omega v (kept as r after a restart)
+            { // (r, r0) lost: FP64 residual, r0 = p = r
+                SpmvArgs<double> a{};
+                a.x = x;
+                a.b = b;
+                a.yo = r;
+                a.yo2 = r0;
+                a.yo3 = p;
+                RedArgs ra = red_args(H);
+                ra.gate = &cs->rstnow;
+                spmv<double>(*A, Prec::F64, EPI_RESID_MX, 1, nv, a, ra, fin64(FIN_MX_RST));
+            }
+            { // p = r + beta p - beta omega v (p = r on a restart)
                 EwArgs<float> a;
                 a.n = A->n;
                 a.z = p;
@@ -871,7 +888,8 @@ class ContinuedRefine : public Solver<double> {
                 a.y = v;
                 a.b = k32->nbo;
                 a.c = k32->beta;
-                a.keep_if = &cs->restart;
+                a.copy_if = &cs->restart;
+                a.keep_if = &cs->rstnow;
                 ew<float>(EW_AXPBYPCZ, nv, a, H);
             }
         };

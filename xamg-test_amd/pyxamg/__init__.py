@@ -122,6 +122,17 @@ def lib():
             "xamgx_hierarchy_get_eigs": (ci, [sz, ci, vp, vp]),
             "xamgx_matrix_set_hierarchy": (ci, [sz, sz]),
             "xamgx_hierarchy_destroy": (ci, [sz]),
+            "xamgx_batch_create": (ci, [ci, vp, vp, vp, vp, vp, psz]),
+            "xamgx_batch_setup": (ci, [sz, sz]),
+            "xamgx_batch_solve": (ci, [sz, sz, sz]),
+            "xamgx_batch_get_info": (ci, [sz, vp, vp, vp]),
+            "xamgx_batch_residuals": (ci, [sz, sz, sz, vp]),
+            "xamgx_batch_last_ms": (ci, [sz, vp]),
+            "xamgx_batch_set_launch": (ci, [ci, ci, ci, ci]),
+            "xamgx_batch_destroy": (ci, [sz]),
+            "xamgx_gen_batch_var5_layout": (ci, [ci, C.c_uint32, ci, ci, ci, vp, vp]),
+            "xamgx_gen_batch_var5_nnz": (C.c_int64, [ci, vp]),
+            "xamgx_gen_batch_var5": (ci, [ci, vp, C.c_uint32, vp, vp, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -351,6 +362,74 @@ class Hierarchy:
 
     def destroy(self):
         check(lib().xamgx_hierarchy_destroy(self.id))
+
+
+class Batch:
+    """Independent systems solved together (xamgx_batch_*): concatenated block-diagonal
+    CSR with global indices, first row of each system, nv per system."""
+
+    def __init__(self, sys_rows, nv, row, col, val):
+        sys_rows = np.ascontiguousarray(sys_rows, dtype=np.int32)
+        self.nv = np.ascontiguousarray(nv, dtype=np.int32)
+        row, col, val = _csr_arrays(row, col, val)
+        self.nsys = len(self.nv)
+        self.sys_rows = sys_rows
+        self.nelem = int(np.sum(np.diff(sys_rows).astype(np.int64) * self.nv))
+        h = C.c_size_t()
+        check(lib().xamgx_batch_create(self.nsys, _p(sys_rows), _p(self.nv), _p(row), _p(col),
+                                       _p(val), C.byref(h)))
+        self.id = h.value
+
+    def setup(self, params):
+        P = params if isinstance(params, Params) else Params(params)
+        check(lib().xamgx_batch_setup(self.id, P.id))
+        if P is not params:
+            P.destroy()
+
+    def vector(self):
+        return Vector(self.nelem, 1)
+
+    def solve(self, x, b):
+        check(lib().xamgx_batch_solve(self.id, x.id, b.id))
+
+    def info(self):
+        it = np.zeros(self.nsys, dtype=np.int32)
+        ck = np.zeros(self.nsys, dtype=np.int32)
+        cv = np.zeros(int(self.nv.sum()), dtype=np.int32)
+        check(lib().xamgx_batch_get_info(self.id, _p(it), _p(ck), _p(cv)))
+        return it, ck, cv.astype(bool)
+
+    def residuals(self, x, b):
+        out = np.zeros(int(self.nv.sum()))
+        check(lib().xamgx_batch_residuals(self.id, x.id, b.id, _p(out)))
+        return out
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().xamgx_batch_last_ms(self.id, C.byref(ms)))
+        return ms.value
+
+    def destroy(self):
+        check(lib().xamgx_batch_destroy(self.id))
+
+
+def batch_set_launch(grid=0, tile_rows=0, batch=0, use_graph=-1):
+    check(lib().xamgx_batch_set_launch(grid, tile_rows, batch, use_graph))
+
+
+def batch_var5(nsys, seed, smin=16, smax=256, maxnv=4):
+    """Generated batch: (side, nv, sys_rows, row, col, val) of nsys 2D 5-point systems."""
+    side = np.zeros(nsys, dtype=np.int32)
+    nv = np.zeros(nsys, dtype=np.int32)
+    check(lib().xamgx_gen_batch_var5_layout(nsys, seed, smin, smax, maxnv, _p(side), _p(nv)))
+    nnz = lib().xamgx_gen_batch_var5_nnz(nsys, _p(side))
+    n = int(np.sum(side.astype(np.int64) ** 2))
+    sys_rows = np.zeros(nsys + 1, dtype=np.int32)
+    row = np.zeros(n + 1, dtype=np.int32)
+    col = np.zeros(nnz, dtype=np.int32)
+    val = np.zeros(nnz)
+    check(lib().xamgx_gen_batch_var5(nsys, _p(side), seed, _p(sys_rows), _p(row), _p(col), _p(val)))
+    return side, nv, sys_rows, row, col, val
 
 
 class Solver:
