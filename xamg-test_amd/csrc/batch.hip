@@ -89,19 +89,52 @@ __device__ void tile_partial(double (&acc)[ND][kBatchMaxNV], double *out) {
     __syncthreads();
 }
 
-// y = epilogue(A x) over the tiles of the running systems; csr_matvec.inl:213-231 row
-// sums in ascending column order, x * val.
+// Element (row, RHS) of a tile: q = local row * nv + l.  Division by nv in {1..4}
+// without a divide instruction.
+__device__ __forceinline__ uint32_t div_nv(uint32_t q, int nv) {
+    if (nv == 1)
+        return q;
+    if (nv == 2)
+        return q >> 1;
+    if (nv == 4)
+        return q >> 2;
+    return (uint32_t)(((uint64_t)q * 0xAAAAAAABull) >> 33); // exact q / 3 for 32-bit q
+}
+
+template <typename T>
+__device__ __forceinline__ T sel4(const T (&v)[kBatchMaxNV], int l) {
+    T r = v[0];
+    r = l == 1 ? v[1] : r;
+    r = l == 2 ? v[2] : r;
+    r = l == 3 ? v[3] : r;
+    return r;
+}
+
+// Adds v to slot l of a per-RHS accumulator (predicated, no dynamic register index).
+__device__ __forceinline__ void acc_slot(double (&acc)[kBatchMaxNV], int l, double v) {
+#pragma unroll
+    for (int q = 0; q < kBatchMaxNV; q++)
+        if (q == l)
+            acc[q] += v;
+}
+
+// y = epilogue(A x) over the tiles of the running systems.  One lane per vector element
+// (row, RHS): lanes of one row read the same column / value entries; the row's nonzeros
+// are loaded U at a time and summed in ascending column order (csr_matvec.inl:213-231,
+// x * val).
 template <typename V, typename T, int EPI, int ND>
 __global__ void __launch_bounds__(BT)
     kb_spmv(BLayout L, const int *halt, BSpmvArgs<T> a, double *part) {
+    constexpr int U = 8;
     const V *val = (const V *)a.val;
     for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
         const int s = L.tsys[t];
         if ((halt != nullptr && halt[s]) || (a.gate != nullptr && !a.gate[s]))
             continue;
         const int nv = L.nv[s];
-        const int64_t vb = L.voff[s] - (int64_t)L.srow[s] * nv;
-        const uint32_t r0 = L.trow[t], r1 = L.trow[t + 1];
+        const uint32_t rs = L.srow[s];
+        const int64_t vb = L.voff[s] - (int64_t)rs * nv;
+        const uint32_t q0 = (L.trow[t] - rs) * nv, q1 = (L.trow[t + 1] - rs) * nv;
         const bool cp = a.copy_if == nullptr || a.copy_if[s];
         constexpr int NDX = ND > 0 ? ND : 1;
         double dacc[NDX][kBatchMaxNV];
@@ -110,56 +143,63 @@ __global__ void __launch_bounds__(BT)
 #pragma unroll
             for (int l = 0; l < kBatchMaxNV; l++)
                 dacc[k][l] = 0.0;
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += BT) {
-            T acc[kBatchMaxNV] = {(T)0, (T)0, (T)0, (T)0};
-            const uint32_t j1 = L.row[i + 1];
-            for (uint32_t j = L.row[i]; j < j1; j++) {
-                const T v = (T)val[j];
-                const T *xp = a.x + vb + (int64_t)L.col[j] * nv;
+        for (uint32_t q = q0 + threadIdx.x; q < q1; q += BT) {
+            const uint32_t li = div_nv(q, nv);
+            const int l = (int)(q - li * nv);
+            const uint32_t i = rs + li;
+            const int64_t e = vb + (int64_t)rs * nv + q;
+            const T *xb = a.x + vb + l;
+            T acc = (T)0;
+            const uint32_t j0 = L.row[i], j1 = L.row[i + 1];
+            for (uint32_t j = j0; j < j1; j += U) {
+                uint32_t c[U];
+                T v[U], xv[U];
 #pragma unroll
-                for (int l = 0; l < kBatchMaxNV; l++)
-                    if (l < nv)
-                        acc[l] = acc[l] + xp[l] * v;
+                for (int u = 0; u < U; u++) {
+                    const bool ok = j + u < j1;
+                    c[u] = ok ? L.col[j + u] : L.col[j0];
+                    v[u] = ok ? (T)val[j + u] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    xv[u] = xb[(int64_t)c[u] * nv];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j + u < j1)
+                        acc = acc + xv[u] * v[u];
             }
-            const int64_t e = vb + (int64_t)i * nv;
-#pragma unroll
-            for (int l = 0; l < kBatchMaxNV; l++) {
-                if (l >= nv)
-                    continue;
-                T out;
-                if constexpr (EPI == EPI_AXY) {
-                    out = acc[l];
-                } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
-                    out = (T)1.0 * a.b[e + l] + (T)(-1.0) * acc[l];
-                } else { // EPI_JACOBI: x_new = x + (w d)(b - A x)
-                    out = a.xin[e + l] + (a.wcoef[s * kBatchMaxNV + l] * a.d[i]) *
-                                             (a.b[e + l] - acc[l]);
+            T out;
+            if constexpr (EPI == EPI_AXY) {
+                out = acc;
+            } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
+                out = (T)1.0 * a.b[e] + (T)(-1.0) * acc;
+            } else { // EPI_JACOBI: x_new = x + (w d)(b - A x)
+                out = a.xin[e] + (a.wcoef[s * kBatchMaxNV + l] * a.d[i]) * (a.b[e] - acc);
+            }
+            if constexpr (EPI == EPI_RESID_MX) {
+                const float o = (float)out;
+                a.yo[e] = o;
+                if (cp) {
+                    if (a.yo2 != nullptr)
+                        a.yo2[e] = o;
+                    if (a.yo3 != nullptr)
+                        a.yo3[e] = o;
                 }
-                if constexpr (EPI == EPI_RESID_MX) {
-                    const float o = (float)out;
-                    a.yo[e + l] = o;
-                    if (cp) {
-                        if (a.yo2 != nullptr)
-                            a.yo2[e + l] = o;
-                        if (a.yo3 != nullptr)
-                            a.yo3[e + l] = o;
-                    }
-                } else {
-                    a.y[e + l] = out;
-                    if (a.y2 != nullptr)
-                        a.y2[e + l] = out;
-                    if (a.y3 != nullptr)
-                        a.y3[e + l] = out;
-                }
-                if constexpr (ND == 1) {
-                    const double wv = a.w != nullptr ? (double)a.w[e + l] : (double)out;
-                    dacc[0][l] += (double)out * wv;
-                } else if constexpr (ND == 3) {
-                    const double o = (double)out, wv = (double)a.w[e + l];
-                    dacc[0][l] += o * wv;
-                    dacc[1][l] += o * o;
-                    dacc[2][l] += wv * wv;
-                }
+            } else {
+                a.y[e] = out;
+                if (a.y2 != nullptr)
+                    a.y2[e] = out;
+                if (a.y3 != nullptr)
+                    a.y3[e] = out;
+            }
+            if constexpr (ND == 1) {
+                const double wv = a.w != nullptr ? (double)a.w[e] : (double)out;
+                acc_slot(dacc[0], l, (double)out * wv);
+            } else if constexpr (ND == 3) {
+                const double o = (double)out, wv = (double)a.w[e];
+                acc_slot(dacc[0], l, o * wv);
+                acc_slot(dacc[1], l, o * o);
+                acc_slot(dacc[2], l, wv * wv);
             }
         }
         if constexpr (ND > 0)
@@ -168,7 +208,7 @@ __global__ void __launch_bounds__(BT)
 }
 
 // Element-wise updates over the tiles of the running systems (blas.inl / blas_ext.inl
-// operation order, per-(system, RHS) coefficients).
+// operation order, per-(system, RHS) coefficients), one lane per vector element.
 template <typename T, int OP, int ND>
 __global__ void __launch_bounds__(BT)
     kb_ew(BLayout L, const int *halt, BEwArgs<T> a, double *part) {
@@ -179,21 +219,30 @@ __global__ void __launch_bounds__(BT)
         const bool keep = a.keep != nullptr && a.keep[s];
         const bool copy = a.copy != nullptr && a.copy[s];
         const int nv = L.nv[s];
-        const int64_t vb = L.voff[s] - (int64_t)L.srow[s] * nv;
-        const uint32_t r0 = L.trow[t], r1 = L.trow[t + 1];
+        const uint32_t rs = L.srow[s];
+        const int64_t eb = L.voff[s];
+        const uint32_t q0 = (L.trow[t] - rs) * nv, q1 = (L.trow[t + 1] - rs) * nv;
         T ca[kBatchMaxNV], cb[kBatchMaxNV], cc[kBatchMaxNV];
         bool frozen[kBatchMaxNV];
 #pragma unroll
         for (int l = 0; l < kBatchMaxNV; l++) {
-            const int q = s * kBatchMaxNV + l;
-            ca[l] = a.ca != nullptr ? a.ca[q] : (T)1.0;
-            cb[l] = a.cb != nullptr ? a.cb[q] : (T)1.0;
-            cc[l] = a.cc != nullptr ? a.cc[q] : (T)1.0;
+            const int qq = s * kBatchMaxNV + l;
+            ca[l] = a.ca != nullptr ? a.ca[qq] : (T)1.0;
+            cb[l] = a.cb != nullptr ? a.cb[qq] : (T)1.0;
+            cc[l] = a.cc != nullptr ? a.cc[qq] : (T)1.0;
             frozen[l] = false;
             if constexpr (OP == EW_AXPBYPCZ)
                 frozen[l] = a.freeze_zero && ca[l] == (T)0 && cb[l] == (T)0;
             else if constexpr (OP == EW_MX_XUPD)
-                frozen[l] = a.freeze_zero && a.ao[q] == 0.0f && a.bo[q] == 0.0f;
+                frozen[l] = a.freeze_zero && a.ao[qq] == 0.0f && a.bo[qq] == 0.0f;
+        }
+        float ao[kBatchMaxNV], bo[kBatchMaxNV];
+        if constexpr (OP == EW_MX_XUPD) {
+#pragma unroll
+            for (int l = 0; l < kBatchMaxNV; l++) {
+                ao[l] = a.ao[s * kBatchMaxNV + l];
+                bo[l] = a.bo[s * kBatchMaxNV + l];
+            }
         }
         constexpr int NDX = ND > 0 ? ND : 1;
         double dacc[NDX][kBatchMaxNV];
@@ -202,51 +251,47 @@ __global__ void __launch_bounds__(BT)
 #pragma unroll
             for (int l = 0; l < kBatchMaxNV; l++)
                 dacc[k][l] = 0.0;
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += BT) {
-            const int64_t e0 = vb + (int64_t)i * nv;
-#pragma unroll
-            for (int l = 0; l < kBatchMaxNV; l++) {
-                if (l >= nv)
-                    continue;
-                const int64_t e = e0 + l;
-                T out;
-                if constexpr (OP == EW_FILL) {
-                    out = a.c0;
-                } else if constexpr (OP == EW_AXPBY_Z) {
-                    out = keep ? a.z[e] : ca[l] * a.x[e] + cb[l] * a.y[e];
-                    if (copy && a.y2 != nullptr)
-                        a.y2[e] = out;
-                } else if constexpr (OP == EW_AXPBYPCZ) {
-                    if (copy)
-                        out = a.x[e];
-                    else
-                        out = (keep || frozen[l])
-                                  ? a.z[e]
-                                  : ca[l] * a.x[e] + cb[l] * a.y[e] + cc[l] * a.z[e];
-                } else if constexpr (OP == EW_JACOBI0) {
-                    out = (ca[l] * a.d[i]) * a.x[e];
-                } else if constexpr (OP == EW_COPY3) {
-                    out = (T)1.0 * a.x[e];
-                    if (a.y2 != nullptr)
-                        a.y2[e] = out;
-                    if (a.y3 != nullptr)
-                        a.y3[e] = out;
-                } else { // EW_MX_XUPD
-                    const int q = s * kBatchMaxNV + l;
-                    out = frozen[l] ? a.z[e]
-                                    : (T)a.ao[q] * (T)((const float *)a.xo)[e] +
-                                          (T)a.bo[q] * (T)((const float *)a.yo)[e] + (T)1.0 * a.z[e];
-                }
-                a.z[e] = out;
-                if constexpr (ND == 1) {
-                    const double wv = (a.w != nullptr && !copy) ? (double)a.w[e] : (double)out;
-                    dacc[0][l] += (double)out * wv;
-                } else if constexpr (ND == 3) {
-                    const double o = (double)out, wv = copy ? o : (double)a.w[e];
-                    dacc[0][l] += o * wv;
-                    dacc[1][l] += o * o;
-                    dacc[2][l] += wv * wv;
-                }
+        for (uint32_t q = q0 + threadIdx.x; q < q1; q += BT) {
+            const uint32_t li = div_nv(q, nv);
+            const int l = (int)(q - li * nv);
+            const int64_t e = eb + q;
+            T out;
+            if constexpr (OP == EW_FILL) {
+                out = a.c0;
+            } else if constexpr (OP == EW_AXPBY_Z) {
+                out = keep ? a.z[e] : sel4(ca, l) * a.x[e] + sel4(cb, l) * a.y[e];
+                if (copy && a.y2 != nullptr)
+                    a.y2[e] = out;
+            } else if constexpr (OP == EW_AXPBYPCZ) {
+                if (copy)
+                    out = a.x[e];
+                else
+                    out = (keep || sel4(frozen, l))
+                              ? a.z[e]
+                              : sel4(ca, l) * a.x[e] + sel4(cb, l) * a.y[e] + sel4(cc, l) * a.z[e];
+            } else if constexpr (OP == EW_JACOBI0) {
+                out = (sel4(ca, l) * a.d[rs + li]) * a.x[e];
+            } else if constexpr (OP == EW_COPY3) {
+                out = (T)1.0 * a.x[e];
+                if (a.y2 != nullptr)
+                    a.y2[e] = out;
+                if (a.y3 != nullptr)
+                    a.y3[e] = out;
+            } else { // EW_MX_XUPD
+                out = sel4(frozen, l) ? a.z[e]
+                                      : (T)sel4(ao, l) * (T)((const float *)a.xo)[e] +
+                                            (T)sel4(bo, l) * (T)((const float *)a.yo)[e] +
+                                            (T)1.0 * a.z[e];
+            }
+            a.z[e] = out;
+            if constexpr (ND == 1) {
+                const double wv = (a.w != nullptr && !copy) ? (double)a.w[e] : (double)out;
+                acc_slot(dacc[0], l, (double)out * wv);
+            } else if constexpr (ND == 3) {
+                const double o = (double)out, wv = copy ? o : (double)a.w[e];
+                acc_slot(dacc[0], l, o * wv);
+                acc_slot(dacc[1], l, o * o);
+                acc_slot(dacc[2], l, wv * wv);
             }
         }
         if constexpr (ND > 0)
@@ -305,8 +350,9 @@ __device__ void bhalt(BState<F> S, int ns, int s) {
 // one thread per system (pbicgstab.inl:108-204 scalar steps; ir.inl / residual
 // replacement for the FP32 recurrences).  F is the precision of the recurrences.
 template <typename F>
-__global__ void kb_finish(BLayout L, BState<F> S, BFinArgs f) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) kb_finish(BLayout L, BState<F> S, BFinArgs f) {
+    // one wavefront per system: lanes sum interleaved tiles, then a fixed xor tree
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int ns = L.nsys;
     if (s >= ns)
         return;
@@ -319,15 +365,29 @@ __global__ void kb_finish(BLayout L, BState<F> S, BFinArgs f) {
         return;
     const int nv = L.nv[s];
     double sums[kBatchDots][kBatchMaxNV];
+#pragma unroll
     for (int k = 0; k < kBatchDots; k++)
+#pragma unroll
         for (int l = 0; l < kBatchMaxNV; l++)
             sums[k][l] = 0.0;
-    for (int t = L.stile[s]; t < L.stile[s + 1]; t++) {
+    for (int t = L.stile[s] + lane; t < L.stile[s + 1]; t += 64) {
         const double *p = S.part + (int64_t)t * PSTRIDE;
-        for (int k = 0; k < f.nd; k++)
-            for (int l = 0; l < nv; l++)
-                sums[k][l] += p[k * kBatchMaxNV + l];
+#pragma unroll
+        for (int k = 0; k < kBatchDots; k++)
+#pragma unroll
+            for (int l = 0; l < kBatchMaxNV; l++)
+                if (k < f.nd && l < nv)
+                    sums[k][l] += p[k * kBatchMaxNV + l];
     }
+#pragma unroll
+    for (int k = 0; k < kBatchDots; k++)
+#pragma unroll
+        for (int l = 0; l < kBatchMaxNV; l++)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                sums[k][l] += __shfl_xor(sums[k][l], off, 64);
+    if (lane != 0)
+        return;
     const int NS4 = ns * kBatchMaxNV, q0 = s * kBatchMaxNV;
     auto SC = [&](int fld, int l) -> F & { return S.sc[fld * NS4 + q0 + l]; };
     F v[kBatchMaxNV];
@@ -671,7 +731,7 @@ void launch_bew(const BLayout &L, const int *halt, const BEwArgs<T> &a, double *
 }
 template <typename F>
 void launch_bfin(const BLayout &L, const BState<F> &S, const BFinArgs &f, hipStream_t st) {
-    hipLaunchKernelGGL((kb_finish<F>), dim3((L.nsys + 63) / 64), dim3(64), 0, st, L, S, f);
+    hipLaunchKernelGGL((kb_finish<F>), dim3((L.nsys + 3) / 4), dim3(256), 0, st, L, S, f);
 }
 
 template <typename T>
