@@ -106,6 +106,8 @@ def main():
                          "recurrence residual by this factor (0: off)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay batches of solver iterations as captured hipGraphs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--spmv-only", action="store_true",
@@ -120,7 +122,7 @@ def main():
 
     import pyxamg as X
     X.init()
-    X.set_engine(batch=args.batch)
+    X.set_engine(batch=args.batch, use_graphs=int(args.graphs))
     dev = X.device_info()
 
     n1 = args.n

@@ -24,7 +24,46 @@ static std::set<std::string> split_set(const std::string &s) {
     return out;
 }
 
+namespace {
+// Set while a batch of iterations is being captured into a hipGraph: a loop nested in
+// the captured body cannot poll the device, so it aborts the capture.
+thread_local bool g_capturing = false;
+struct CaptureAbort {};
+
+// Captures k calls of body() on the runtime stream; returns null when the body cannot
+// be captured (it runs a polling loop of its own).
+hipGraphExec_t capture_batch(int k, const std::function<void()> &body) {
+    hipStream_t st = Runtime::get().stream;
+    XAMG_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    g_capturing = true;
+    bool ok = true;
+    try {
+        for (int i = 0; i < k; i++)
+            body();
+    } catch (const CaptureAbort &) {
+        ok = false;
+    } catch (...) {
+        g_capturing = false;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g)
+            (void)hipGraphDestroy(g);
+        throw;
+    }
+    g_capturing = false;
+    hipGraph_t g = nullptr;
+    XAMG_HIP_CHECK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    if (ok)
+        XAMG_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    return ex;
+}
+} // namespace
+
 int run_loop(int max_iters, ConvState *cs, const std::function<void()> &body) {
+    if (g_capturing)
+        throw CaptureAbort{};
     Runtime &rt = Runtime::get();
     static hipEvent_t ev[2] = {nullptr, nullptr};
     if (!ev[0]) {
@@ -32,11 +71,25 @@ int run_loop(int max_iters, ConvState *cs, const std::function<void()> &body) {
         XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     }
     int K = std::max(1, engine_cfg().batch);
+    // hipGraph replay of full batches.  The graph is captured once per loop, at the
+    // second batch: by then every lazy setup (autotuning, dense inverses) has run and a
+    // body's first-iteration special cases (a zero initial guess) are behind it, so the
+    // captured iterations are the ones every later batch repeats.
+    hipGraphExec_t gx = nullptr;
+    bool tried = false;
     int it = 0, batches = 0, pending = -1;
     while (it < max_iters) {
         int k = std::min(K, max_iters - it);
-        for (int i = 0; i < k; i++)
-            body();
+        if (engine_cfg().use_graphs && k == K && batches >= 1 && !tried) {
+            tried = true;
+            gx = capture_batch(K, body);
+        }
+        if (gx != nullptr && k == K) {
+            XAMG_HIP_CHECK(hipGraphLaunch(gx, rt.stream));
+        } else {
+            for (int i = 0; i < k; i++)
+                body();
+        }
         it += k;
         batches++;
         if (cs == nullptr)
@@ -54,6 +107,10 @@ int run_loop(int max_iters, ConvState *cs, const std::function<void()> &body) {
     }
     if (cs != nullptr && pending >= 0)
         XAMG_HIP_CHECK(hipEventSynchronize(ev[pending]));
+    if (gx != nullptr) {
+        XAMG_HIP_CHECK(hipStreamSynchronize(rt.stream));
+        (void)hipGraphExecDestroy(gx);
+    }
     return batches;
 }
 
