@@ -203,7 +203,10 @@ def test_ir_fp32_inner_fp64_outer(xamg, inner):
 
 
 @pytest.mark.parametrize("prec", [None, {"method": "Jacobi"},
-                                  {"method": "Chebyshev", "polynomial_order": 4}])
+                                  {"method": "Chebyshev", "polynomial_order": 4},
+                                  {"method": "Chebyshev", "polynomial_order": 4,
+                                   "scaled_operator": False},
+                                  {"method": "Chebyshev", "polynomial_order": 2}])
 def test_continued_refinement_fp32_krylov_fp64_residual(xamg, prec):
     """IterativeRefinement with refinement=continued: FP32 (P)BiCGStab recurrences with
     FP64 residual replacement.  No reference counterpart (the reference restarts the
@@ -235,7 +238,7 @@ def test_continued_refinement_fp32_krylov_fp64_residual(xamg, prec):
     p64 = {"solver": {"method": method, "rel_tolerance": 1e-10, "abs_tolerance": 0.0,
                       "max_iters": 1000}}
     if prec:
-        p64["preconditioner"] = dict(prec)
+        p64["preconditioner"] = {k: v for k, v in prec.items() if k != "scaled_operator"}
     orc = O.solve(p64, O.csr(row, col, val), b, nv, eigs=eigs)
     outer, inner = gpu["iters"], gpu["sub_iters"]
     assert 1 <= outer <= 20

@@ -63,23 +63,60 @@ std::vector<int> vpt_options(int nv) {
     }
 }
 
+// Timing buffers shaped like the epilogue's operands (contents irrelevant: ones).
 template <typename T>
-float time_cfg(DevMatrix &A, Prec vp, int nv, const T *x, T *y, int vpt, int grid, int remap) {
+struct TuneBufs {
+    DBuf x, y, b, w, d, wc, yo;
+    TuneBufs(const DevMatrix &A, int nv) {
+        const int64_t n = std::max<int64_t>(A.n, 1), nc = std::max<int64_t>(A.ncols, 1);
+        x.alloc(sizeof(T) * nc * nv);
+        y.alloc(sizeof(T) * n * nv);
+        b.alloc(sizeof(T) * n * nv);
+        w.alloc(sizeof(T) * n * nv);
+        d.alloc(sizeof(T) * nc);
+        wc.alloc(sizeof(T) * kMaxNV);
+        yo.alloc(sizeof(float) * n * nv);
+        fill<T>(x.as<T>(), nc, nv, (T)1);
+        fill<T>(b.as<T>(), n, nv, (T)1);
+        fill<T>(w.as<T>(), n, nv, (T)1);
+        fill<T>(d.as<T>(), nc, 1, (T)1);
+        fill<T>(wc.as<T>(), 1, kMaxNV, (T)1);
+    }
+    SpmvArgs<T> args(int epi) {
+        SpmvArgs<T> a{};
+        a.x = x.as<T>();
+        a.y = y.as<T>();
+        a.w = w.as<T>();
+        if (epi != EPI_AXY && epi != EPI_AXPY) {
+            a.b = b.as<T>();
+            a.d = d.as<T>();
+            a.dcol = d.as<T>();
+            a.xin = x.as<T>();
+            a.wcoef = wc.as<T>();
+            a.coef = (T)0.5;
+            a.xcoef = (T)0.5;
+        }
+        if (epi == EPI_RESID_MX)
+            a.yo = yo.as<float>();
+        return a;
+    }
+};
+
+template <typename T>
+float time_cfg(DevMatrix &A, Prec vp, int nv, int epi, int nd, SpmvArgs<T> a, int vpt, int grid,
+               int remap) {
     hipStream_t s = Runtime::get().stream;
-    SpmvArgs<T> a{};
-    a.x = x;
-    a.y = y;
     a.cfg_vpt = vpt;
     a.cfg_grid = grid;
     a.cfg_remap = remap;
     hipEvent_t e0, e1;
     XAMG_HIP_CHECK(hipEventCreate(&e0));
     XAMG_HIP_CHECK(hipEventCreate(&e1));
-    spmv<T>(A, vp, EPI_AXY, 0, nv, a, red_args(), FinArgs<T>{}); // warm-up
+    spmv<T>(A, vp, epi, nd, nv, a, red_args(), FinArgs<T>{}); // warm-up
     XAMG_HIP_CHECK(hipEventRecord(e0, s));
     const int reps = 4;
     for (int i = 0; i < reps; i++)
-        spmv<T>(A, vp, EPI_AXY, 0, nv, a, red_args(), FinArgs<T>{});
+        spmv<T>(A, vp, epi, nd, nv, a, red_args(), FinArgs<T>{});
     XAMG_HIP_CHECK(hipEventRecord(e1, s));
     XAMG_HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;
@@ -90,16 +127,15 @@ float time_cfg(DevMatrix &A, Prec vp, int nv, const T *x, T *y, int vpt, int gri
 }
 
 template <typename T>
-DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv) {
-    // x has one row per column of A (the transfer operators of a hierarchy are rectangular)
-    DBuf x(sizeof(T) * std::max<int64_t>(A.ncols, 1) * nv), y(sizeof(T) * std::max<int64_t>(A.n, 1) * nv);
-    fill<T>(x.as<T>(), A.ncols, nv, (T)1);
+DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
+    TuneBufs<T> bufs(A, nv);
+    const SpmvArgs<T> a = bufs.args(epi);
     DevMatrix::Tune best;
     float best_ms = 1e30f;
     for (int vpt : vpt_options(nv))
         for (int grid : {1024, 2048, 4096, 8192})
             for (int remap : {0, 1}) {
-                float ms = time_cfg<T>(A, vp, nv, x.as<T>(), y.as<T>(), vpt, grid, remap);
+                float ms = time_cfg<T>(A, vp, nv, epi, nd, a, vpt, grid, remap);
                 if (ms < best_ms) {
                     best_ms = ms;
                     best.vpt = vpt;
@@ -113,8 +149,9 @@ DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv) {
 
 } // namespace
 
-const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv) {
-    Tune &t = tune[(int)vp][es == 8 ? 0 : 1][nv];
+const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd) {
+    const int slot = (((int)vp * 2 + (es == 8 ? 0 : 1)) * (kMaxNV + 1) + nv) * 64 + epi * 4 + nd;
+    Tune &t = tune[slot];
     if (t.valid)
         return t;
     t.valid = true; // defaults unless tuned below
@@ -122,7 +159,8 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv) {
     if (!autotune_enabled() || n < 65536)
         return t;
     std::ostringstream k;
-    k << "n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv" << nv;
+    k << "n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv" << nv
+      << "_epi" << epi << "_nd" << nd;
     const std::string key = k.str();
     auto &c = cache();
     auto it = c.find(key);
@@ -130,11 +168,12 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv) {
         t = it->second;
         return t;
     }
-    Tune r = es == 8 ? run_tuning<double>(*this, vp, nv) : run_tuning<float>(*this, vp, nv);
-    t = r;
+    Tune r = es == 8 ? run_tuning<double>(*this, vp, nv, epi, nd)
+                     : run_tuning<float>(*this, vp, nv, epi, nd);
+    tune[slot] = r;
     c[key] = r;
     save(key, r);
-    return t;
+    return tune[slot];
 }
 
 } // namespace xamg

@@ -38,9 +38,11 @@ struct HipError : std::runtime_error {
             throw std::runtime_error(std::string("xamg: ") + (msg));                              \
     } while (0)
 
-enum class Prec : int { F64 = 0, F32 = 1, BF16 = 2 };
+// F32S: FP32 values of D^{-1/2} A D^{-1/2} (the symmetrically scaled operator whose
+// polynomial the Chebyshev smoother evaluates), formed in FP64 and rounded once.
+enum class Prec : int { F64 = 0, F32 = 1, BF16 = 2, F32S = 3 };
 
-inline size_t prec_size(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
+inline size_t prec_size(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::BF16 ? 2 : 4); }
 
 // Device-resident convergence record of one solver instance (convergence.h:49-303).
 // Lives in HBM; kernels update it, the host reads it back only at batch ends.

@@ -189,12 +189,12 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
         for (int l = 0; l < nv; l++) {
             F s2 = (F)sums[l], s3 = (F)sums[nv + l], s4 = (F)sums[2 * nv + l];
             F om = s2 / s3;
+            F res = om * s2;
+            res = (F)1.0 * s4 + (F)(-1.0) * res;
             ks->omega[l] = om;
             ks->nomega[l] = om * (F)-1.0;
             ks->ac[l] = mask_coef(ks->iconv[l], ks->alpha[l]);
             ks->oc[l] = mask_coef(ks->iconv[l], om);
-            F res = om * s2;
-            res = (F)1.0 * s4 + (F)(-1.0) * res;
             v[l] = res;
             if (!cs->status[l]) {
                 const double rr = (double)res;
@@ -447,6 +447,28 @@ __global__ void __launch_bounds__(kBlock)
                 for (int v = 0; v < VPT; v++)
                     acc[v] = (T)0;
             }
+            // the epilogue's own row operands are loaded before the gathers so their
+            // latency overlaps the nonzero loop
+            constexpr bool kNeedB = EPI != EPI_AXY && EPI != EPI_AXPY;
+            constexpr bool kNeedD = EPI == EPI_JACOBI || EPI == EPI_CHEB || EPI == EPI_CHEB_LAST;
+            T pb[VPT], pw[VPT], px[VPT];
+            T pdi = (T)0;
+            if constexpr (kNeedB)
+                ld_vec<T, VPT>(a.b + e, pb);
+            if constexpr (kNeedD)
+                pdi = a.d[i];
+            if constexpr (EPI == EPI_JACOBI) {
+                ld_vec<T, VPT>(a.xin + e, px);
+            } else if constexpr (EPI == EPI_CHEB_LAST) {
+                if (a.xin != nullptr)
+                    ld_vec<T, VPT>(a.xin + e, px);
+            }
+            if constexpr (ND == 3) {
+                ld_vec<T, VPT>(a.w + e, pw);
+            } else if constexpr (ND == 1) {
+                if (a.w != nullptr)
+                    ld_vec<T, VPT>(a.w + e, pw);
+            }
             for (uint32_t j = j0; j < j1; j += U) {
                 uint32_t c[U];
                 T av[U];
@@ -461,7 +483,16 @@ __global__ void __launch_bounds__(kBlock)
                 for (int u = 0; u < U; u++)
                     ld_vec<T, VPT>(x + (int64_t)c[u] * NV + l0, xv[u]);
                 if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
-                    if (a.x_is_rhs) {
+                    if (a.x_is_rhs && a.prescaled) {
+                        // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const T dc = a.dcol[c[u]];
+#pragma unroll
+                            for (int v = 0; v < VPT; v++)
+                                xv[u][v] = a.xcoef * (xv[u][v] * dc);
+                        }
+                    } else if (a.x_is_rhs) {
                         // first Horner step read straight from the right-hand side:
                         // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
 #pragma unroll
@@ -471,7 +502,7 @@ __global__ void __launch_bounds__(kBlock)
                             for (int v = 0; v < VPT; v++)
                                 xv[u][v] = (a.xcoef * (xv[u][v] * dc)) * dc;
                         }
-                    } else {
+                    } else if (!a.prescaled) {
 #pragma unroll
                         for (int u = 0; u < U; u++) {
                             const T dc = a.dcol[c[u]];
@@ -494,27 +525,20 @@ __global__ void __launch_bounds__(kBlock)
                 for (int v = 0; v < VPT; v++)
                     out[v] = acc[v];
             } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
-                T bb[VPT];
-                ld_vec<T, VPT>(a.b + e, bb);
 #pragma unroll
                 for (int v = 0; v < VPT; v++)
-                    out[v] = (T)1.0 * bb[v] + (T)(-1.0) * acc[v];
+                    out[v] = (T)1.0 * pb[v] + (T)(-1.0) * acc[v];
             } else if constexpr (EPI == EPI_JACOBI) {
-                T bb[VPT], xi[VPT];
-                ld_vec<T, VPT>(a.b + e, bb);
-                ld_vec<T, VPT>(a.xin + e, xi);
-                const T di = a.d[i];
+                const T di = pdi;
 #pragma unroll
                 for (int v = 0; v < VPT; v++)
-                    out[v] = xi[v] + (a.wcoef[l0 + v] * di) * (bb[v] - acc[v]);
+                    out[v] = px[v] + (a.wcoef[l0 + v] * di) * (pb[v] - acc[v]);
             } else {
-                T bb[VPT];
-                ld_vec<T, VPT>(a.b + e, bb);
-                const T di = a.d[i];
+                const T di = pdi;
 #pragma unroll
                 for (int v = 0; v < VPT; v++) {
-                    const T rr = bb[v] * di;
-                    const T t2 = acc[v] * di;
+                    const T rr = pb[v] * di;
+                    const T t2 = a.prescaled ? acc[v] : acc[v] * di;
                     out[v] = a.coef * rr + (T)1.0 * t2;
                 }
                 if constexpr (EPI == EPI_CHEB_LAST) {
@@ -522,11 +546,9 @@ __global__ void __launch_bounds__(kBlock)
                     for (int v = 0; v < VPT; v++)
                         out[v] = out[v] * di;
                     if (a.xin != nullptr) {
-                        T xi[VPT];
-                        ld_vec<T, VPT>(a.xin + e, xi);
 #pragma unroll
                         for (int v = 0; v < VPT; v++)
-                            out[v] = (T)1.0 * out[v] + (T)1.0 * xi[v];
+                            out[v] = (T)1.0 * out[v] + (T)1.0 * px[v];
                     }
                 }
             }
@@ -549,22 +571,18 @@ __global__ void __launch_bounds__(kBlock)
             }
             if constexpr (ND == 1) {
                 if (a.w != nullptr) {
-                    T ww[VPT];
-                    ld_vec<T, VPT>(a.w + e, ww);
 #pragma unroll
                     for (int v = 0; v < VPT; v++)
-                        dacc[0][v] += (double)out[v] * (double)ww[v];
+                        dacc[0][v] += (double)out[v] * (double)pw[v];
                 } else {
 #pragma unroll
                     for (int v = 0; v < VPT; v++)
                         dacc[0][v] += (double)out[v] * (double)out[v];
                 }
             } else if constexpr (ND == 3) {
-                T ww[VPT];
-                ld_vec<T, VPT>(a.w + e, ww);
 #pragma unroll
                 for (int v = 0; v < VPT; v++) {
-                    const double o = (double)out[v], wv = (double)ww[v];
+                    const double o = (double)out[v], wv = (double)pw[v];
                     dacc[0][v] += o * wv;
                     dacc[1][v] += o * o;
                     dacc[2][v] += wv * wv;

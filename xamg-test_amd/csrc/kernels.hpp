@@ -106,6 +106,7 @@ struct SpmvArgs {
     T coef = 0;              // CHEB coefficient c_q
     int x_is_rhs = 0;        // CHEB: x is the (unscaled) RHS of the first Horner step
     T xcoef = 0;             // CHEB: leading coefficient c_{k-1} applied to that RHS
+    int prescaled = 0;       // CHEB: values hold D^{-1/2} A D^{-1/2} (no column / row scaling)
     const T *wcoef = nullptr; // JACOBI per-RHS relax factors
     const T *w = nullptr;    // second operand of the fused dot(s)
     T *y2 = nullptr, *y3 = nullptr; // extra copies of y (initial residual: r, p)

@@ -79,6 +79,22 @@ const void *DevMatrix::values(Prec p) {
         }
         return val32.p;
     }
+    if (p == Prec::F32S) {
+        if (!val32s.p) {
+            std::vector<double> d(std::max<int64_t>(n, 1), 1.0);
+            for (int64_t i = 0; i < n; i++)
+                for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
+                    if (hcol[j] == (uint32_t)i)
+                        d[i] = std::sqrt(1.0 / hval[j]);
+            std::vector<float> v(std::max<int64_t>(nnz, 1));
+            for (int64_t i = 0; i < n; i++)
+                for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
+                    v[j] = (float)((d[i] * hval[j]) * d[hcol[j]]);
+            val32s.alloc(sizeof(float) * v.size());
+            XAMG_HIP_CHECK(hipMemcpy(val32s.p, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
+        }
+        return val32s.p;
+    }
     if (!valbf.p) {
         std::vector<uint16_t> v(std::max<int64_t>(nnz, 1));
         for (int64_t j = 0; j < nnz; j++)
@@ -201,7 +217,9 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
     a.col = A.col.as<uint32_t>();
     a.n = A.n;
     if (a.cfg_vpt == 0 && a.cfg_grid == 0) {
-        const DevMatrix::Tune &t = A.tuned(vp, (int)sizeof(T), nv);
+        // the scaled FP32 values share the FP32 launch configuration (same bytes)
+        const DevMatrix::Tune &t =
+            A.tuned(vp == Prec::F32S ? Prec::F32 : vp, (int)sizeof(T), nv, epi, nd);
         a.cfg_vpt = t.vpt;
         a.cfg_grid = t.grid;
         a.cfg_remap = t.remap;
@@ -214,6 +232,10 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
         if (vp == Prec::BF16) {
             a.val = A.values(Prec::BF16);
             launch_spmv<__hip_bfloat16, float>(epi, nd, nv, a, r, f, s);
+        } else if (vp == Prec::F32S) {
+            a.val = A.values(Prec::F32S);
+            a.prescaled = 1;
+            launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
         } else {
             a.val = A.values(Prec::F32);
             launch_spmv<float, float>(epi, nd, nv, a, r, f, s);

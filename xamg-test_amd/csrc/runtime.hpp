@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <limits>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -77,7 +78,7 @@ struct DevMatrix {
     int64_t n = 0, ncols = 0, nnz = 0;
     std::vector<uint32_t> hrow, hcol;
     std::vector<double> hval;
-    DBuf row, col, val64, val32, valbf;
+    DBuf row, col, val64, val32, valbf, val32s;
     DBuf inv_diag64, inv_sqrt_diag64, inv_diag32, inv_sqrt_diag32;
     DBuf dense_inv64, dense_inv32; // Direct: column-major inverse (inverse.inl:108-168)
     double min_eig = std::numeric_limits<double>::quiet_NaN();
@@ -98,14 +99,15 @@ struct DevMatrix {
     // caller supplies no bounds; hypre's setup supplies them in the reference).
     void estimate_eigs(int steps = 60);
 
-    // SpMV launch configuration per (value precision, vector element size, nv), chosen
-    // by timing the candidates on this matrix (autotune.cpp).
+    // SpMV launch configuration per (value precision, vector element size, nv,
+    // epilogue, fused dots), chosen by timing the candidates on this matrix with the
+    // epilogue's own extra streams (autotune.hip).
     struct Tune {
         int vpt = 0, grid = 0, remap = -1;
         bool valid = false;
     };
-    Tune tune[3][2][kMaxNV + 1];
-    const Tune &tuned(Prec vp, int es, int nv);
+    std::map<int, Tune> tune;
+    const Tune &tuned(Prec vp, int es, int nv, int epi, int nd);
 };
 
 // Autotuner switches: XAMG_AUTOTUNE=0 disables it; XAMG_TUNE_FILE names a text file

@@ -365,6 +365,9 @@ template <typename T>
 class Chebyshev : public Solver<T> {
     T c[4];
     int order = 2;
+    // precision of the Horner steps' matrix values: the solver's own, or FP32 values of
+    // D^{-1/2} A D^{-1/2} (scaled_operator, FP32 only) so the steps skip the scaling
+    Prec hprec = Prec::F64;
 
   public:
     void setup() override {
@@ -372,6 +375,12 @@ class Chebyshev : public Solver<T> {
         this->alloc_bufs(3);
         auto *A = this->A;
         A->template inv_sqrt_diag<T>();
+        hprec = this->vprec;
+        if (std::is_same<T, float>::value && this->vprec == Prec::F32 &&
+            this->prm.has("scaled_operator") && this->prm.get_bool("scaled_operator")) {
+            hprec = Prec::F32S;
+            A->values(Prec::F32S);
+        }
         order = (int)this->prm.get_int("polynomial_order");
         XAMG_REQUIRE(order >= 1 && order <= 4, "Chebyshev polynomial_order must be 1..4");
         double lo = A->min_eig, hi = A->max_eig;
@@ -465,12 +474,12 @@ class Chebyshev : public Solver<T> {
                 s.coef = c[q];
                 if (q > 0) {
                     s.y = t2;
-                    spmv<T>(*A, this->vprec, EPI_CHEB, 0, nv, s, red_args(H), FinArgs<T>{});
+                    spmv<T>(*A, hprec, EPI_CHEB, 0, nv, s, red_args(H), FinArgs<T>{});
                     std::swap(t1, t2);
                 } else {
                     s.y = x;
                     s.xin = xzero ? nullptr : x;
-                    spmv<T>(*A, this->vprec, EPI_CHEB_LAST, 0, nv, s, red_args(H), FinArgs<T>{});
+                    spmv<T>(*A, hprec, EPI_CHEB_LAST, 0, nv, s, red_args(H), FinArgs<T>{});
                 }
             }
         }
@@ -771,7 +780,7 @@ class ContinuedRefine : public Solver<double> {
     DBuf st32;
     KrylovScalars<float> *k32 = nullptr;
     double delta2 = 1e-4;
-    double theta = 1e-7;
+    double theta = 0.0;
     double gap2 = 100.0;
     int max_inner = 1000;
 
@@ -789,7 +798,7 @@ class ContinuedRefine : public Solver<double> {
         max_inner = sp.has("max_iters") ? (int)sp.get_int("max_iters") : 1000;
         double step = prm.has("replace_step") ? prm.get_float("replace_step") : 1e-2;
         delta2 = step * step;
-        theta = prm.has("restart_threshold") ? prm.get_float("restart_threshold") : 1e-7;
+        theta = prm.has("restart_threshold") ? prm.get_float("restart_threshold") : 0.0;
         const double gap = prm.has("restart_gap") ? prm.get_float("restart_gap") : 10.0;
         gap2 = gap * gap;
         if (!conv_check)
@@ -806,7 +815,11 @@ class ContinuedRefine : public Solver<double> {
         A->values(Prec::F32);
         if (pre) {
             XAMG_REQUIRE(gp->has("preconditioner"), "PBiCGStab needs a preconditioner block");
-            const ParamList &pp = gp->get("preconditioner");
+            ParamList pp = gp->get("preconditioner");
+            // the FP32 Chebyshev smoother of the continued iteration evaluates its
+            // polynomial on FP32 values of D^{-1/2} A D^{-1/2} unless told otherwise
+            if (pp.get_string("method") == "Chebyshev" && !pp.has("scaled_operator"))
+                pp.set("scaled_operator", "1");
             precond = make_solver<float>(pp.get_string("method"), A, pp, "preconditioner", nv, gp, 0);
         }
     }
@@ -924,9 +937,9 @@ class ContinuedRefine : public Solver<double> {
                 FinArgs<float> fa = fin32(FIN_BETA);
                 fa.clear_on_fin = &cs->replaced;
                 fa.theta = theta;
-                ew<float>(EW_AXPBY_Z, 3, nv, a, ra, fa);
+                ew<float>(EW_AXPBY_Z, theta > 0 ? 3 : 1, nv, a, ra, fa);
             }
-            { // (r, r0) lost: FP64 residual, r0 = p = r
+            if (theta > 0) { // (r, r0) lost: FP64 residual, r0 = p = r
                 SpmvArgs<double> a{};
                 a.x = x;
                 a.b = b;
