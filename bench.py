@@ -6,13 +6,14 @@ CSR with int32 indices, nv=8 right-hand sides uniform[-1,1] from seed 2, x0 = 0,
 PBiCGStab + Chebyshev(4) to a 1e-10 relative residual (abs_tolerance 0).
 
 A step is one complete solve to tolerance with the FP32-inner / FP64-refined method
-(IterativeRefinement around an FP32 PBiCGStab + FP32 Chebyshev), inputs resident in
-HBM.  The plain FP64 GPU solve of the same system is timed the same way; `value` is
-the mixed-precision time to tolerance in seconds (lower is better) and the line also
-carries the FP64 time, the speedup, the final FP64-verified residuals and the error
-against an FP64 reference solution, the SpMV bandwidth with its roofline record, and
-the CPU baseline (the oracle's OpenMP restatement of the reference path, timed on a
-bounded sample of iterations on the host cores).
+(IterativeRefinement with refinement=continued: FP32 PBiCGStab recurrences and FP32
+Chebyshev, FP64 solution and residual replacement), inputs resident in HBM.  The plain
+FP64 GPU solve of the same system is timed the same way; `value` is the mixed-precision
+time to tolerance in seconds (lower is better) and the line also carries the FP64 time,
+the speedup, the final FP64-verified residuals and the error against the FP64 solution,
+the SpMV bandwidth with its roofline record, the CPU baseline (the oracle's OpenMP
+restatement of the reference path, timed on a bounded sample of iterations on the host
+cores) and time-to-tolerance records of the other workloads (other_configs).
 """
 import argparse
 import json
@@ -23,6 +24,8 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# the library's residual tables would interleave with the one JSON line
+os.environ.setdefault("XAMG_LOG", "quiet")
 sys.path[:0] = [ROOT, os.path.join(ROOT, "xamg-test_amd")]
 
 HBM_PEAK_GBS = 8000.0
@@ -87,6 +90,153 @@ def cpu_baseline(row, col, val, b, nv, kind, gpu_iters, eigs, budget_s=15.0):
                       f"({t:.1f} s), scaled to the {gpu_iters} iterations of the GPU FP64 solve"}
 
 
+def _timed_solves(X, A, xv, bv, params, reps=2, warm=1):
+    """Mean wall time of `reps` solves (after `warm` untimed ones) from x = 0, with the
+    iteration record and the solution of the last one."""
+    P = X.Params(params)
+    P.set_default()
+    S = X.Solver(A, P, xv, bv)
+    for _ in range(warm):
+        xv.set_val(0.0)
+        S.solve()
+    ts = []
+    for _ in range(reps):
+        xv.set_val(0.0)
+        X.check(X.lib().XAMG_barrier())
+        t0 = time.perf_counter()
+        S.solve()
+        X.check(X.lib().XAMG_barrier())
+        ts.append(time.perf_counter() - t0)
+    out = {"s": float(np.mean(ts)), "iters": list(S.iters()), "converged": bool(all(S.converged()))}
+    S.destroy()
+    P.destroy()
+    return out, xv.download()
+
+
+def _relres(A, x, b, n, nv):
+    r = (b - A.spmv(x, nv, prec=0)).reshape(n, nv)
+    return float((np.linalg.norm(r, axis=0) / np.linalg.norm(b.reshape(n, nv), axis=0)).max())
+
+
+def other_configs(X, tol):
+    """Time-to-tolerance on the other workloads (FP64 against the FP32-inner variants),
+    each verified with the FP64 SpMV.  Reported beside the headline, not part of it."""
+    out = {}
+    mixed_meta = {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
+                  "rel_tolerance": tol, "abs_tolerance": 0.0, "refinement": "continued"}
+
+    def fp64(method, prec, t=tol, max_iters=20000):
+        p = {"solver": {"method": method, "rel_tolerance": t, "abs_tolerance": 0.0,
+                        "max_iters": max_iters}}
+        if prec:
+            p["preconditioner"] = dict(prec)
+        return p
+
+    def mixed(method, prec, max_iters=20000):
+        p = {"meta_solver": dict(mixed_meta),
+             "solver": {"method": method, "rel_tolerance": 1e-4, "abs_tolerance": 0.0,
+                        "max_iters": max_iters}}
+        if prec:
+            p["preconditioner"] = dict(prec)
+        return p
+
+    def run_pair(name, row, col, val, nv, seed, cases, eigs=None, hier=None):
+        n = len(row) - 1
+        A = X.Matrix(row, col, val)
+        if eigs is not None:
+            A.set_eigs(*eigs)
+        if hier is not None:
+            hier.attach(A)
+        b = X.rhs(n, nv, seed)
+        xv, bv = X.Vector(n, nv), X.Vector(n, nv)
+        bv.upload(b)
+        rec = {}
+        xref = None
+        for label, params in cases:
+            r, x = _timed_solves(X, A, xv, bv, params)
+            r["relres_verified"] = _relres(A, x, b, n, nv)
+            if xref is None:
+                xref = x
+            else:
+                r["err_vs_first"] = float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
+            rec[label] = r
+        first = cases[0][0]
+        for label, _ in cases[1:]:
+            rec[label]["speedup_vs_" + first] = rec[first]["s"] / rec[label]["s"]
+        for o in (xv, bv, A):
+            o.destroy()
+        out[name] = rec
+
+    # 7-point 32^3, nv=1, PBiCGStab + Jacobi to 1e-8 (the CPU-runnable case)
+    row, col, val = X.poisson7(32)
+    run_pair("7pt-poisson 32^3 nv=1 pbicgstab+jacobi rtol=1e-8", row, col, val, 1, 1,
+             [("fp64", fp64("PBiCGStab", {"method": "Jacobi"}, t=1e-8))])
+    # 7-point 128^3 at the other RHS counts, PBiCGStab + Chebyshev(4)
+    row, col, val = X.poisson7(128)
+    c = np.cos(np.pi / 129)
+    cheb = {"method": "Chebyshev", "polynomial_order": 4, "spectrum_fraction": 0.3}
+    for nv in (1, 4, 16):
+        run_pair(f"7pt-poisson 128^3 nv={nv} pbicgstab+cheb4", row, col, val, nv, 2,
+                 [("fp64", fp64("PBiCGStab", cheb)), ("fp32_continued", mixed("PBiCGStab", cheb))],
+                 eigs=(1 - c, 1 + c))
+    # 27-point anisotropic 160^3, nv=8: FP32 values (and BF16 values for the smoother)
+    row, col, val = X.aniso27(160, 160, 160, seed=3)
+    jac = {"method": "Jacobi"}
+    run_pair("27pt-aniso 160^3 nv=8 pbicgstab+jacobi", row, col, val, 8, 3,
+             [("fp64", fp64("PBiCGStab", jac)), ("fp32_continued", mixed("PBiCGStab", jac)),
+              ("fp32_continued_bf16_smoother",
+               mixed("PBiCGStab", dict(jac, values_precision="bf16")))])
+    del row, col, val
+    # 7-point 192^3, nv=16, PBiCGStab + classical AMG V(1,1), Chebyshev smoothers
+    row, col, val = X.poisson7(192)
+    mgb = {"method": "MultiGrid", "mg_strength_threshold": 0.25, "mg_agg_num_levels": 0}
+    t0 = time.perf_counter()
+    H = X.Hierarchy.build_csr(X.Params({"preconditioner": mgb}), row, col, val)
+    setup_s = time.perf_counter() - t0
+    mg = {"solver": {"method": "PBiCGStab", "rel_tolerance": tol, "abs_tolerance": 0.0,
+                     "max_iters": 200},
+          "pre_smoother": {"method": "Chebyshev"}, "post_smoother": {"method": "Chebyshev"},
+          "coarse_grid_solver": {"method": "Direct"}}
+    mgr = json.loads(json.dumps(mg))
+    mg["preconditioner"] = dict(mgb)
+    mgr["preconditioner"] = dict(mgb, mg_reduced_precision=True)
+    c = np.cos(np.pi / 193)
+    run_pair("7pt-poisson 192^3 nv=16 pbicgstab+amg", row, col, val, 16, 4,
+             [("fp64_levels", mg), ("fp32_levels", mgr)], eigs=(1 - c, 1 + c), hier=H)
+    out["7pt-poisson 192^3 nv=16 pbicgstab+amg"]["host_setup_s"] = setup_s
+    out["7pt-poisson 192^3 nv=16 pbicgstab+amg"]["levels"] = H.levels()
+    H.destroy()
+    del row, col, val
+    # 2,000 independent 5-point systems, side log-uniform [16, 256], 1-4 RHS each
+    side, nvk, sr, row, col, val = X.batch_var5(2000, 5, smin=16, smax=256, maxnv=4)
+    B = X.Batch(sr, nvk, row, col, val)
+    x, bv = B.vector(), B.vector()
+    b = X.rhs(B.nelem, 1, 5)
+    bv.upload(b)
+    rec = {}
+    for label, params in (("fp64", fp64("PBiCGStab", jac, max_iters=5000)),
+                          ("fp32_continued", mixed("PBiCGStab", jac, max_iters=5000))):
+        B.setup(params)
+        ts = []
+        for rep in range(2):
+            x.upload(np.zeros(B.nelem))
+            t0 = time.perf_counter()
+            B.solve(x, bv)
+            ts.append(time.perf_counter() - t0)
+        it, ck, conv = B.info()
+        rr = B.residuals(x, bv)
+        rec[label] = {"s": float(min(ts)), "device_ms": B.last_ms(), "max_iters": int(it.max()),
+                      "median_iters": float(np.median(it)), "converged": bool(conv.all()),
+                      "relres_verified": float(rr.max())}
+    rec["fp32_continued"]["speedup_vs_fp64"] = rec["fp64"]["s"] / rec["fp32_continued"]["s"]
+    rec["rows"] = int(sr[-1])
+    rec["nnz"] = int(len(col))
+    for o in (x, bv, B):
+        o.destroy()
+    out["batch 2000 x 5pt var-coef side 16-256 nv 1-4"] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +256,8 @@ def main():
                          "recurrence residual by this factor (0: off)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the time-to-tolerance records of the other workloads")
     ap.add_argument("--graphs", action="store_true",
                     help="replay batches of solver iterations as captured hipGraphs")
     ap.add_argument("--no-cpu", action="store_true")
@@ -250,6 +402,10 @@ def main():
         line["cpu_baseline"] = cpu_baseline(row, col, val, b, nv, args.solver, it64, eigs,
                                             budget_s=args.cpu_budget)
         line["cpu_baseline"]["gpu_speedup_fp64"] = line["cpu_baseline"]["value"] / mean_64
+    if not args.headline_only:
+        for o in (xv, bv, A):
+            o.destroy()
+        line["other_configs"] = other_configs(X, args.tol)
     print(json.dumps(line), flush=True)
 
 
