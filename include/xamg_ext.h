@@ -113,6 +113,26 @@ int xamgx_hierarchy_get_eigs(XAMG_hierarchy_id hid, int level, double *lo, doubl
 int xamgx_matrix_set_hierarchy(XAMG_matrix_id mid, XAMG_hierarchy_id hid);
 int xamgx_hierarchy_destroy(XAMG_hierarchy_id hid);
 
+/* ---- system files and result records ----
+ * Binary system file of the reference's test drivers (mtx_gen.cpp:116-147 writes it,
+ * reader.inl:47-177 reads it): u32 nrows, u32 nnz, u32 row[nrows+1], u32 col[nnz],
+ * f64 val[nnz], then optionally f64 b[nrows] and f64 x[nrows].  A file without b reads
+ * as b = 1 and one without x as x = 0 (the reference reader's fallbacks). */
+int xamgx_read_system_info(const char *path, uint64_t *nrows, uint64_t *nnz, int *has_b, int *has_x);
+/* Arrays of nrows+1, nnz, nnz, nrows, nrows entries; NULL skips one. */
+int xamgx_read_system(const char *path, int *row, int *col, double *val, double *b, double *x);
+/* b and x may be NULL (x only when b is given too). */
+int xamgx_write_system(const char *path, uint64_t nrows, const int *row, const int *col,
+                       const double *val, const double *b, const double *x);
+/* YAML record of the last solve in the layout of the reference's test driver
+ * (tst_solver.h:155-179, tst_output.h:32-91): info (nrows, nonzeros, NV, nprocs,
+ * config), solver (X_initial_norm when x0_norm is given, X_norm, Y_norm, iters,
+ * converged, abs_residual, rel_residual, initial_residual), timing (solver seconds),
+ * topo. */
+int xamgx_solver_write_record(XAMG_solver_id sid, XAMG_vector_id x, XAMG_vector_id b,
+                              const double *x0_norm, double seconds, const char *config,
+                              const char *path);
+
 /* ---- batches of independent systems ----
  * Many independent systems of uneven sizes, each with its own nv (1..4), solved
  * together: the reference solves them one by one (one solver per system, NV fixed at

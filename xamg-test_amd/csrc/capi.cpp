@@ -30,6 +30,14 @@ void batch_var5_layout(int nsys, uint32_t seed, int smin, int smax, int maxnv, i
 int64_t batch_var5_nnz(int nsys, const int *side);
 void batch_var5(int nsys, const int *side, uint32_t seed, int *sys_rows, int *row, int *col,
                 double *val);
+void read_system_info(const char *path, uint64_t *nrows, uint64_t *nnz, int *has_b, int *has_x);
+void read_system(const char *path, int *row, int *col, double *val, double *b, double *x);
+void write_system(const char *path, uint64_t nrows, const int *row, const int *col, const double *val,
+                  const double *b, const double *x);
+void write_solver_record(const char *path, int64_t nrows, int64_t nnz, int nv, const double *x0_norm,
+                         const double *x_norm, const double *b_norm, int iters, const int *converged,
+                         const double *abs_res, const double *rel_res, const double *init_res,
+                         double seconds, const char *config);
 } // namespace xamg
 
 using namespace xamg;
@@ -1138,6 +1146,49 @@ int xamgx_batch_destroy(XAMG_batch_id bid) {
             delete g_batches[bid];
             g_batches[bid] = nullptr;
         }
+    });
+}
+
+} // extern "C"
+
+// ---- system files and result records ----
+extern "C" {
+
+int xamgx_read_system_info(const char *path, uint64_t *nrows, uint64_t *nnz, int *has_b, int *has_x) {
+    return guard([&] { read_system_info(path, nrows, nnz, has_b, has_x); });
+}
+int xamgx_read_system(const char *path, int *row, int *col, double *val, double *b, double *x) {
+    return guard([&] { read_system(path, row, col, val, b, x); });
+}
+int xamgx_write_system(const char *path, uint64_t nrows, const int *row, const int *col,
+                       const double *val, const double *b, const double *x) {
+    return guard([&] { write_system(path, nrows, row, col, val, b, x); });
+}
+int xamgx_solver_write_record(XAMG_solver_id sid, XAMG_vector_id x, XAMG_vector_id b,
+                              const double *x0_norm, double seconds, const char *config,
+                              const char *path) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        DevMatrix *A = slot(g_mats, S->mid, "matrix")->m.get();
+        const int nv = S->nv;
+        std::vector<double> xn(nv), bn(nv);
+        XAMG_REQUIRE(xamgx_dot(x, x, nv, xn.data()) == 0 && xamgx_dot(b, b, nv, bn.data()) == 0,
+                     g_err);
+        SolveStats st;
+        S->top->stats(st, false);
+        std::vector<double> ar(nv), rr(nv), ir(nv);
+        std::vector<int> cv(nv);
+        for (int l = 0; l < nv; l++) {
+            xn[l] = std::sqrt(xn[l]);
+            bn[l] = std::sqrt(bn[l]);
+            // stats.h:134-148: abs = sqrt(res), rel = sqrt(res / res0), initial = sqrt(res0)
+            ar[l] = std::sqrt(st.res[l]);
+            rr[l] = std::sqrt(st.res[l] / st.res0[l]);
+            ir[l] = std::sqrt(st.res0[l]);
+            cv[l] = st.converged[l];
+        }
+        write_solver_record(path, A->n, A->nnz, nv, x0_norm, xn.data(), bn.data(), st.iters,
+                            cv.data(), ar.data(), rr.data(), ir.data(), seconds, config);
     });
 }
 

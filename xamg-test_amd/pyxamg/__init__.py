@@ -133,6 +133,10 @@ def lib():
             "xamgx_gen_batch_var5_layout": (ci, [ci, C.c_uint32, ci, ci, ci, vp, vp]),
             "xamgx_gen_batch_var5_nnz": (C.c_int64, [ci, vp]),
             "xamgx_gen_batch_var5": (ci, [ci, vp, C.c_uint32, vp, vp, vp, vp]),
+            "xamgx_read_system_info": (ci, [cp, vp, vp, vp, vp]),
+            "xamgx_read_system": (ci, [cp, vp, vp, vp, vp, vp]),
+            "xamgx_write_system": (ci, [cp, C.c_uint64, vp, vp, vp, vp, vp]),
+            "xamgx_solver_write_record": (ci, [sz, sz, sz, vp, cd, cp, cp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -413,6 +417,29 @@ class Batch:
         check(lib().xamgx_batch_destroy(self.id))
 
 
+def read_system(path):
+    """Binary system file -> (row, col, val, b, x, has_b, has_x)."""
+    n, nnz = C.c_uint64(), C.c_uint64()
+    hb, hx = C.c_int(), C.c_int()
+    check(lib().xamgx_read_system_info(_b(path), C.byref(n), C.byref(nnz), C.byref(hb), C.byref(hx)))
+    n, nnz = n.value, nnz.value
+    row = np.zeros(n + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz)
+    b = np.zeros(n)
+    x = np.zeros(n)
+    check(lib().xamgx_read_system(_b(path), _p(row), _p(col), _p(val), _p(b), _p(x)))
+    return row, col, val, b, x, bool(hb.value), bool(hx.value)
+
+
+def write_system(path, row, col, val, b=None, x=None):
+    row, col, val = _csr_arrays(row, col, val)
+    b = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+    x = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+    check(lib().xamgx_write_system(_b(path), len(row) - 1, _p(row), _p(col), _p(val),
+                                   None if b is None else _p(b), None if x is None else _p(x)))
+
+
 def batch_set_launch(grid=0, tile_rows=0, batch=0, use_graph=-1):
     check(lib().xamgx_batch_set_launch(grid, tile_rows, batch, use_graph))
 
@@ -476,6 +503,12 @@ class Solver:
         ms = C.c_double()
         check(lib().xamgx_solver_last_ms(self.id, C.byref(ms)))
         return ms.value
+
+    def write_record(self, path, x0_norm=None, seconds=0.0, config=""):
+        x0 = None if x0_norm is None else np.ascontiguousarray(x0_norm, dtype=np.float64)
+        check(lib().xamgx_solver_write_record(self.id, self.x.id, self.b.id,
+                                              None if x0 is None else _p(x0), float(seconds),
+                                              _b(config), _b(path)))
 
     def destroy(self):
         check(lib().XAMG_solver_destroy(self.id))
