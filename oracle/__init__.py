@@ -16,11 +16,12 @@ _LIB = None
 MAX_NV = 16
 MAX_LEVELS = 32
 
-IDENTITY, BICGSTAB, PBICGSTAB, JACOBI, CHEBYSHEV, HSGS, MULTIGRID, DIRECT, IR = range(9)
+IDENTITY, BICGSTAB, PBICGSTAB, JACOBI, CHEBYSHEV, HSGS, MULTIGRID, DIRECT, IR, PCG, RBICGSTAB = \
+    range(11)
 METHODS = {
     "Identity": IDENTITY, "BiCGStab": BICGSTAB, "PBiCGStab": PBICGSTAB, "Jacobi": JACOBI,
     "Chebyshev": CHEBYSHEV, "HSGS": HSGS, "MultiGrid": MULTIGRID, "Direct": DIRECT,
-    "IterativeRefinement": IR,
+    "IterativeRefinement": IR, "PCG": PCG, "RBiCGStab": RBICGSTAB,
 }
 
 
@@ -262,7 +263,7 @@ class Tree:
         n.mg_cycle = int(p.get("mg_cycle", 0))
         n.reduced = int(bool(p.get("subsolver_reduced_precision", False)))
         n.nchunks = self.nchunks
-        if m == "PBiCGStab":
+        if m in ("PBiCGStab", "PCG", "RBiCGStab"):
             n.precond = C.pointer(self._node("preconditioner"))
         if m == "IterativeRefinement":
             n.precond = C.pointer(self._node("solver"))

@@ -91,7 +91,9 @@ enum {
     ORC_HSGS = 5,
     ORC_MULTIGRID = 6,
     ORC_DIRECT = 7,
-    ORC_IR = 8
+    ORC_IR = 8,
+    ORC_PCG = 9,
+    ORC_RBICGSTAB = 10
 };
 
 #define ORC_MAX_LEVELS 32

@@ -382,6 +382,175 @@ static void SFX(solve_bicgstab)(SFX(rt) * rt, F *x, int xzero, const F *b, F *ic
     }
 }
 
+/* ---- PCG pcg.inl:47-157 ---- */
+static void SFX(solve_pcg)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv_in) {
+    int64_t n = rt->n;
+    int nv = rt->nv;
+    const orc_csr *A = &rt->A;
+    F iconv[ORC_MAX_NV], rn[ORC_MAX_NV], s1[ORC_MAX_NV], s2[ORC_MAX_NV], s3[ORC_MAX_NV],
+        alpha[ORC_MAX_NV], beta[ORC_MAX_NV], ac[ORC_MAX_NV];
+    for (int l = 0; l < nv; l++)
+        iconv[l] = iconv_in ? iconv_in[l] : (F)1.0;
+    F *r = rt->buf[0], *z = rt->buf[1], *p = rt->buf[2], *Ap = rt->buf[3];
+    SFX(conv_t) *c = &rt->conv;
+    SFX(conv_init)(c, iconv);
+    /* stats.init(r, iconv); is_converged_initial(r) (stats.h:208-219) */
+    SFX(residual)(A, x, xzero, b, r, nv);
+    if (c->conv_check) {
+        SFX(norm2)(rt, r, rn);
+        SFX(conv_set_initial)(c, rn);
+        if (SFX(conv_is_converged)(c, rn, iconv))
+            return;
+    }
+    /* z = M^{-1} r from a zero-flagged z; p = z */
+    SFX(solve)(rt->precond, z, 1, r, NULL);
+    memcpy(p, z, sizeof(F) * n * nv);
+    if (xzero)
+        memset(x, 0, sizeof(F) * n * nv);
+    int max_iters = rt->node->max_iters;
+    for (int last = 0; !last;) {
+        int it = ++c->iter;
+        if (it >= max_iters)
+            last = 1;
+        SFX(orc_spmv)(A, p, Ap, nv);
+        SFX(orc_dot)(r, z, n, nv, rt->node->nchunks, s1);
+        SFX(orc_dot)(p, Ap, n, nv, rt->node->nchunks, s2);
+        for (int l = 0; l < nv; l++) {
+            alpha[l] = s1[l] / s2[l];
+            ac[l] = iconv[l] * alpha[l];
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                x[i * nv + l] = ac[l] * p[i * nv + l] + (F)1.0 * x[i * nv + l];
+        for (int l = 0; l < nv; l++)
+            alpha[l] = (F)(-1.0) * alpha[l];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                r[i * nv + l] = alpha[l] * Ap[i * nv + l] + (F)1.0 * r[i * nv + l];
+        SFX(solve)(rt->precond, z, 1, r, NULL);
+        SFX(orc_dot)(r, z, n, nv, rt->node->nchunks, s3);
+        for (int l = 0; l < nv; l++)
+            beta[l] = s3[l] / s1[l];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                p[i * nv + l] = (F)1.0 * z[i * nv + l] + beta[l] * p[i * nv + l];
+        SFX(orc_dot)(r, r, n, nv, rt->node->nchunks, rn);
+        if (SFX(conv_is_converged)(c, rn, iconv))
+            return;
+    }
+}
+
+/* ---- RBiCGStab rbicgstab.inl:47-205 ---- */
+static void SFX(solve_rbicgstab)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv_in) {
+    int64_t n = rt->n;
+    int nv = rt->nv;
+    const orc_csr *A = &rt->A;
+    F iconv[ORC_MAX_NV], rho[ORC_MAX_NV], res[ORC_MAX_NV], theta[ORC_MAX_NV], phi[ORC_MAX_NV],
+        psi[ORC_MAX_NV], delta[ORC_MAX_NV], alpha[ORC_MAX_NV], beta[ORC_MAX_NV], omega[ORC_MAX_NV],
+        eta[ORC_MAX_NV], temp[ORC_MAX_NV], ac[ORC_MAX_NV], oc[ORC_MAX_NV];
+    for (int l = 0; l < nv; l++)
+        iconv[l] = iconv_in ? iconv_in[l] : (F)1.0;
+    F *r0 = rt->buf[0], *r = rt->buf[1], *z = rt->buf[2], *v_ = rt->buf[3], *v = rt->buf[4],
+      *s = rt->buf[5], *t_ = rt->buf[6], *t = rt->buf[7];
+    SFX(conv_t) *c = &rt->conv;
+    SFX(conv_init)(c, iconv);
+    /* 1-2: r0 = b - A x0, rho = (r0, r0) (stats.h:195-206) */
+    SFX(residual)(A, x, xzero, b, r0, nv);
+    SFX(norm2)(rt, r0, rho);
+    if (c->conv_check) {
+        SFX(conv_set_initial)(c, rho);
+        if (SFX(conv_is_converged)(c, rho, iconv))
+            return;
+    }
+    memcpy(r, r0, sizeof(F) * n * nv);
+    SFX(solve)(rt->precond, z, 1, r0, NULL); /* 3 */
+    memcpy(v_, z, sizeof(F) * n * nv);      /* 4 */
+    if (xzero)
+        memset(x, 0, sizeof(F) * n * nv);
+    int max_iters = rt->node->max_iters;
+    for (int last = 0; !last;) {
+        int it = ++c->iter;
+        if (it >= max_iters)
+            last = 1;
+        SFX(orc_spmv)(A, v_, v, nv);                          /* 6 */
+        SFX(orc_dot)(v, r0, n, nv, rt->node->nchunks, delta); /* 7 */
+        SFX(solve)(rt->precond, s, 1, v, NULL);               /* 8 */
+        for (int l = 0; l < nv; l++)                          /* 9 */
+            alpha[l] = rho[l] / delta[l];
+        for (int l = 0; l < nv; l++) /* 10: t_ = z - alpha s */
+            alpha[l] = (F)(-1.0) * alpha[l];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                t_[i * nv + l] = (F)1.0 * z[i * nv + l] + alpha[l] * s[i * nv + l];
+        for (int l = 0; l < nv; l++)
+            alpha[l] = (F)(-1.0) * alpha[l];
+        SFX(orc_spmv)(A, t_, t, nv); /* 11 */
+        for (int l = 0; l < nv; l++) /* 12: r = r - alpha v */
+            alpha[l] = (F)(-1.0) * alpha[l];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                r[i * nv + l] = alpha[l] * v[i * nv + l] + (F)1.0 * r[i * nv + l];
+        for (int l = 0; l < nv; l++)
+            alpha[l] = (F)(-1.0) * alpha[l];
+        SFX(orc_dot)(t, r, n, nv, rt->node->nchunks, theta); /* 13 */
+        SFX(orc_dot)(t, t, n, nv, rt->node->nchunks, phi);
+        SFX(orc_dot)(t, r0, n, nv, rt->node->nchunks, psi);
+        SFX(orc_dot)(r, r, n, nv, rt->node->nchunks, eta);
+        SFX(solve)(rt->precond, z, 1, t, NULL); /* 14 */
+        for (int l = 0; l < nv; l++)            /* 15 */
+            omega[l] = theta[l] / phi[l];
+        for (int l = 0; l < nv; l++) /* 16: r = r - omega t */
+            omega[l] = (F)(-1.0) * omega[l];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                r[i * nv + l] = omega[l] * t[i * nv + l] + (F)1.0 * r[i * nv + l];
+        for (int l = 0; l < nv; l++)
+            omega[l] = (F)(-1.0) * omega[l];
+        for (int l = 0; l < nv; l++) { /* 17 */
+            res[l] = omega[l] * theta[l];
+            res[l] = (F)1.0 * eta[l] + (F)(-1.0) * res[l];
+        }
+        for (int l = 0; l < nv; l++) { /* 23: x += alpha v_ + omega t_ */
+            ac[l] = iconv[l] * alpha[l];
+            oc[l] = iconv[l] * omega[l];
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                x[i * nv + l] = ac[l] * v_[i * nv + l] + oc[l] * t_[i * nv + l] +
+                                (F)1.0 * x[i * nv + l];
+        if (SFX(conv_is_converged)(c, res, iconv))
+            return;
+        for (int l = 0; l < nv; l++) { /* 21-22 */
+            rho[l] = omega[l] * psi[l];
+            rho[l] = (F)(-1.0) * rho[l];
+            temp[l] = delta[l] * omega[l];
+            beta[l] = rho[l] / temp[l];
+        }
+        for (int l = 0; l < nv; l++) /* 24: z = t_ - omega z (omega stays negated) */
+            omega[l] = (F)(-1.0) * omega[l];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                z[i * nv + l] = (F)1.0 * t_[i * nv + l] + omega[l] * z[i * nv + l];
+        for (int l = 0; l < nv; l++) { /* 25: v_ = z + beta v_ - beta omega s */
+            delta[l] = omega[l] * beta[l];
+            omega[l] = (F)(-1.0) * omega[l];
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++)
+            for (int l = 0; l < nv; l++)
+                v_[i * nv + l] = (F)1.0 * z[i * nv + l] + delta[l] * s[i * nv + l] +
+                                 beta[l] * v_[i * nv + l];
+    }
+}
+
 /* ---- Jacobi jacobi.inl:51-102 + merged_jacobi blas_merged_jacobi.inl:32-90 ---- */
 static void SFX(solve_jacobi)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv_in) {
     int64_t n = rt->n;
@@ -760,6 +929,12 @@ static void SFX(solve)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv) {
     case ORC_PBICGSTAB:
         SFX(solve_bicgstab)(rt, x, xzero, b, iconv, 1);
         break;
+    case ORC_PCG:
+        SFX(solve_pcg)(rt, x, xzero, b, iconv);
+        break;
+    case ORC_RBICGSTAB:
+        SFX(solve_rbicgstab)(rt, x, xzero, b, iconv);
+        break;
     case ORC_JACOBI:
         SFX(solve_jacobi)(rt, x, xzero, b, iconv);
         break;
@@ -800,6 +975,10 @@ static int SFX(nbufs)(int method) {
         return 6;
     case ORC_PBICGSTAB:
         return 8;
+    case ORC_PCG:
+        return 4;
+    case ORC_RBICGSTAB:
+        return 8;
     case ORC_JACOBI:
         return 1;
     case ORC_CHEBYSHEV:
@@ -833,6 +1012,8 @@ SFX(rt) * SFX(rt_build)(const orc_node *node, const orc_csr *A, int nv, int leve
         rt->buf[k] = SFX(vnew)(rt->n, nv);
     switch (node->method) {
     case ORC_PBICGSTAB:
+    case ORC_PCG:
+    case ORC_RBICGSTAB:
         rt->precond = SFX(rt_build)(node->precond, A, nv, level);
         break;
     case ORC_JACOBI:

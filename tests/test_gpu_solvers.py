@@ -1,5 +1,5 @@
 """Solver parity of the HIP path against the oracle (bicgstab.inl, pbicgstab.inl,
-jacobi.inl, chebyshev.inl, hsgs.inl, ir.inl, convergence.h).
+pcg.inl, rbicgstab.inl, jacobi.inl, chebyshev.inl, hsgs.inl, ir.inl, convergence.h).
 
 Bar: iteration counts within +-2 of the oracle, residual histories matching while the
 residual is well above round-off, the final residual verified in FP64, and the error
@@ -74,6 +74,11 @@ CASES = [
     ("PBiCGStab", {"method": "Chebyshev", "polynomial_order": 4}),
     ("PBiCGStab", {"method": "Chebyshev", "polynomial_order": 2}),
     ("PBiCGStab", {"method": "HSGS", "gs_blocks": 16}),
+    ("PCG", {"method": "Jacobi"}),
+    ("PCG", {"method": "Chebyshev", "polynomial_order": 4}),
+    ("RBiCGStab", {"method": "Jacobi"}),
+    ("RBiCGStab", {"method": "Chebyshev", "polynomial_order": 4}),
+    ("RBiCGStab", {"method": "HSGS", "gs_blocks": 16}),
 ]
 
 

@@ -330,3 +330,89 @@ def test_poisson_spectrum_closed_form():
     ev = np.linalg.eigvalsh(S)
     lo, hi = poisson_eigs(nx)
     assert abs(ev[0] - lo) < 1e-12 and abs(ev[-1] - hi) < 1e-12
+
+
+def _pcg_numpy(A, b, tol, maxit, M):
+    """Textbook preconditioned conjugate gradients; history of ||r||^2 per check."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = M(r)
+    p = z.copy()
+    rz = r @ z
+    hist = [r @ r]
+    tol2 = (tol * np.linalg.norm(b)) ** 2
+    for it in range(1, maxit + 1):
+        Ap = A @ p
+        alpha = rz / (p @ Ap)
+        x += alpha * p
+        r -= alpha * Ap
+        z = M(r)
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+        hist.append(r @ r)
+        if hist[-1] <= tol2:
+            break
+    return x, it, np.array(hist)
+
+
+@pytest.mark.parametrize("prec", ["Jacobi", "Chebyshev"])
+def test_pcg_against_textbook(prec):
+    """pcg.inl restated: same iterates as textbook PCG with the same (linear, symmetric)
+    preconditioner operator."""
+    nx = 10
+    A = poisson7_scipy(nx)
+    n = A.shape[0]
+    b = O.fill_rhs(n, 1, 4)
+    eigs = poisson_eigs(nx)
+    if prec == "Jacobi":
+        Dinv = 0.99 / A.diagonal()
+
+        def M(v):
+            x1 = Dinv * v
+            return x1 + Dinv * (v - A @ x1)
+    else:
+        def M(v):
+            # Chebyshev(2) preconditioner as a matrix polynomial in D^-1/2 A D^-1/2
+            c = O.cheb_coeffs(eigs[0], eigs[1], 2, 0.3)
+            d = 1 / np.sqrt(A.diagonal())
+            rr = d * v
+            t = c[1] * rr
+            t = c[0] * rr + d * (A @ (d * t))
+            return d * t
+    xt, itt, rest = _pcg_numpy(A, b, 1e-10, 300, M)
+    p = {"solver": {"method": "PCG", "rel_tolerance": 1e-10, "abs_tolerance": 0.0,
+                    "max_iters": 300},
+         "preconditioner": {"method": prec}}
+    r = O.solve(p, O.from_scipy(A), b, 1, eigs=eigs)
+    assert r["converged"] == [True]
+    assert abs(r["iters"] - itt) <= 1
+    k = min(len(rest), len(r["hist"]))
+    assert np.allclose(r["hist"][:k, 0], rest[:k], rtol=1e-7)
+    assert np.linalg.norm(r["x"] - xt) / np.linalg.norm(xt) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["Jacobi", "Chebyshev"])
+@pytest.mark.parametrize("nv", [1, 3])
+def test_rbicgstab_equals_pbicgstab_in_exact_arithmetic(prec, nv):
+    """rbicgstab.inl reorders right-preconditioned BiCGStab (it carries M^-1 p as v_ and
+    updates it by recurrence); with a linear preconditioner the two produce the same
+    residuals up to rounding, and the same solution."""
+    nx = 10
+    A = poisson7_scipy(nx)
+    n = A.shape[0]
+    b = O.fill_rhs(n, nv, 5)
+    eigs = poisson_eigs(nx)
+    common = {"rel_tolerance": 1e-10, "abs_tolerance": 0.0, "max_iters": 300}
+    rb = O.solve({"solver": dict(common, method="RBiCGStab"), "preconditioner": {"method": prec}},
+                 O.from_scipy(A), b, nv, eigs=eigs)
+    pb = O.solve({"solver": dict(common, method="PBiCGStab"), "preconditioner": {"method": prec}},
+                 O.from_scipy(A), b, nv, eigs=eigs)
+    assert rb["converged"] == [True] * nv
+    assert abs(rb["iters"] - pb["iters"]) <= 1
+    k = min(len(rb["hist"]), len(pb["hist"]), 10)
+    for l in range(nv):
+        assert np.allclose(rb["hist"][:k, l], pb["hist"][:k, l], rtol=1e-6)
+    X = rb["x"].reshape(n, nv)
+    xs = spl.spsolve(A.tocsc(), b.reshape(n, nv)).reshape(n, nv)
+    assert np.linalg.norm(X - xs) / np.linalg.norm(xs) < 1e-8

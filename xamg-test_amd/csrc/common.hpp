@@ -15,7 +15,7 @@ namespace xamg {
 constexpr int kMaxNV = 16;
 constexpr int kBlock = 256;    // threads per workgroup for streaming kernels
 constexpr int kNumXcd = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
-constexpr int kMaxDots = 3;    // fused reductions per kernel (phi, psi, theta)
+constexpr int kMaxDots = 4;    // fused reductions per kernel (RBiCGStab: theta, phi, psi, eta)
 
 struct HipError : std::runtime_error {
     explicit HipError(const std::string &s) : std::runtime_error(s) {}

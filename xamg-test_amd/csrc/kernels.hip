@@ -149,6 +149,60 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
                 cs->rstnow = 1;
         }
         break;
+    case FIN_PCG_RZ:
+        for (int l = 0; l < nv; l++)
+            ks->rho[l] = (F)sums[l];
+        break;
+    case FIN_PCG_ALPHA:
+        cs->iter++;
+        for (int l = 0; l < nv; l++) {
+            const F al = ks->rho[l] / (F)sums[l];
+            ks->alpha[l] = al;
+            ks->ac[l] = mask_coef(ks->iconv[l], al);
+            ks->nalpha[l] = (F)(-1.0) * al;
+        }
+        break;
+    case FIN_PCG_BETA:
+        for (int l = 0; l < nv; l++) {
+            const F s3 = (F)sums[l];
+            ks->beta[l] = s3 / ks->rho[l];
+            ks->rho[l] = s3;
+            v[l] = (F)sums[nv + l];
+        }
+        conv_check(f, v, nv, 0);
+        break;
+    case FIN_RB_ALPHA:
+        cs->iter++;
+        for (int l = 0; l < nv; l++) {
+            const F de = (F)sums[l];
+            ks->s1[l] = de;
+            const F al = ks->rho[l] / de;
+            ks->alpha[l] = al;
+            ks->nalpha[l] = (F)(-1.0) * al;
+        }
+        break;
+    case FIN_RB_OMEGA:
+        for (int l = 0; l < nv; l++) {
+            const F th = (F)sums[l], ph = (F)sums[nv + l], ps = (F)sums[2 * nv + l],
+                    et = (F)sums[3 * nv + l];
+            const F om = th / ph;
+            ks->omega[l] = om;
+            ks->nomega[l] = (F)(-1.0) * om;
+            F res = om * th;
+            res = (F)1.0 * et + (F)(-1.0) * res;
+            v[l] = res;
+            ks->ac[l] = mask_coef(ks->iconv[l], ks->alpha[l]);
+            ks->oc[l] = mask_coef(ks->iconv[l], om);
+            F rho = om * ps;
+            rho = (F)(-1.0) * rho;
+            const F temp = ks->s1[l] * om;
+            const F be = rho / temp;
+            ks->rho[l] = rho;
+            ks->beta[l] = be;
+            ks->nbo[l] = ((F)(-1.0) * om) * be;
+        }
+        conv_check(f, v, nv, 0);
+        break;
     case FIN_NORM_INIT:
         for (int l = 0; l < nv; l++)
             v[l] = (F)sums[l];
@@ -359,6 +413,8 @@ static void launch_finish(int nd, const RedArgs &r, const FinArgs<T> &f, int G, 
         hipLaunchKernelGGL((k_finish<T, NV, 1>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
     else if (nd == 3)
         hipLaunchKernelGGL((k_finish<T, NV, 3>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
+    else if (nd == 4)
+        hipLaunchKernelGGL((k_finish<T, NV, 4>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
 }
 
 // Loads / stores of VPT consecutive vector elements (16-byte accesses where possible).
@@ -623,6 +679,8 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
     bool frozen = false;
     if constexpr (OP == EW_AXPBYPCZ)
         frozen = a.freeze_zero && ca == (T)0 && cb == (T)0;
+    else if constexpr (OP == EW_AXPBY_Z)
+        frozen = a.freeze_zero && ca == (T)0;
     else if constexpr (OP == EW_MX_XUPD)
         frozen = a.freeze_zero && a.ao[l] == 0.0f && a.bo[l] == 0.0f;
     double dacc[ND > 0 ? ND : 1][1];
@@ -633,8 +691,8 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
         T out;
         if constexpr (OP == EW_FILL) {
             out = a.c0;
-        } else if constexpr (OP == EW_AXPBY_Z) {
-            out = keep ? a.z[e] : ca * a.x[e] + cb * a.y[e];
+        } else if constexpr (OP == EW_AXPBY_Z || OP == EW_RB_R) {
+            out = (keep || frozen) ? a.z[e] : ca * a.x[e] + cb * a.y[e];
             if (copy && a.y2 != nullptr)
                 a.y2[e] = out;
         } else if constexpr (OP == EW_AXPBYPCZ) {
@@ -656,7 +714,8 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
             }
         } else if constexpr (OP == EW_COPY3) {
             out = (T)1.0 * a.x[e];
-            a.y2[e] = out;
+            if (a.y2 != nullptr)
+                a.y2[e] = out;
             if (a.y3 != nullptr)
                 a.y3[e] = out;
         } else if constexpr (OP == EW_ADD_CONV) {
@@ -669,13 +728,22 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
             out = a.z[e] * a.d[e / NV];
         } else if constexpr (OP == EW_JAC_UPD) {
             out = (ca * cc) * (a.y[e] * a.d[e / NV]) + (T)1.0 * a.z[e];
+        } else if constexpr (OP == EW_DOTS) {
+            out = a.x[e];
         } else { // EW_MX_XUPD: FP64 accumulation of FP32 directions
             out = frozen ? a.z[e]
                          : (T)a.ao[l] * (T)((const float *)a.xo)[e] +
                                (T)a.bo[l] * (T)((const float *)a.yo)[e] + (T)1.0 * a.z[e];
         }
-        a.z[e] = out;
-        if constexpr (ND == 1) {
+        if constexpr (OP != EW_DOTS)
+            a.z[e] = out;
+        if constexpr (OP == EW_RB_R) {
+            const double o = (double)out, wv = (double)a.w[e], w2 = (double)a.w2[e];
+            dacc[0][0] += wv * o;
+            dacc[1][0] += wv * wv;
+            dacc[2][0] += wv * w2;
+            dacc[3][0] += o * o;
+        } else if constexpr (ND == 1) {
             const double wv = (a.w != nullptr && !copy) ? (double)a.w[e] : (double)out;
             dacc[0][0] += (double)out * wv;
         } else if constexpr (ND == 3) {
@@ -902,6 +970,8 @@ static void ew_nv(int op, int nd, const EwArgs<T> &a, const RedArgs &r, const Fi
     XAMG_EW_CASE(EW_AXPBY_Z, 0)
     XAMG_EW_CASE(EW_AXPBY_Z, 1)
     XAMG_EW_CASE(EW_AXPBY_Z, 3)
+    XAMG_EW_CASE(EW_DOTS, 3)
+    XAMG_EW_CASE(EW_RB_R, 4)
     XAMG_EW_CASE(EW_AXPBYPCZ, 0)
     XAMG_EW_CASE(EW_JACOBI0, 0)
     XAMG_EW_CASE(EW_CHEB0, 0)

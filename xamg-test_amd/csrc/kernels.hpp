@@ -41,6 +41,8 @@ enum Ew : int {
     EW_SCAL_D = 9,     // z = z * d_i  (row scaling)                  (blas.inl:173-183)
     EW_JAC_UPD = 10,   // z = (a*c)*(y*d_i) + 1*z  (check-mode Jacobi, jacobi.inl:80-84)
     EW_MX_XUPD = 11,   // z(FP64) = ao*(T)xo + bo*(T)yo + 1*z, FP32 directions/coefficients
+    EW_DOTS = 12,      // no store; dots (x, w), (x, x), (w, w)       (pcg.inl:125-126, 145)
+    EW_RB_R = 13,      // z = a*x + b*y; dots (w,z), (w,w), (w,w2), (z,z) (rbicgstab.inl:113-134)
 };
 
 // Scalar update applied by the last workgroup after a fused reduction.
@@ -58,8 +60,13 @@ enum Fin : int {
     FIN_MX_OMEGA = 9, // as FIN_OMEGA, but a recurrence residual below the tolerance or
                       // below delta^2 * (last FP64 residual) requests an FP64 check
     FIN_MX_REPL = 10, // FP64 true residual: per-RHS convergence, masks (aux), restart ref
-    FIN_MX_RST = 11   // FP64 true residual of a restart (r0 = p = r): as FIN_MX_REPL, and
+    FIN_MX_RST = 11,  // FP64 true residual of a restart (r0 = p = r): as FIN_MX_REPL, and
                       // rho = (r, r); halts at once when every RHS has converged
+    FIN_PCG_RZ = 12,    // rho = (r, z)                                       (pcg.inl:96-99)
+    FIN_PCG_ALPHA = 13, // alpha = rho / (p, Ap), masked alpha, -alpha         (pcg.inl:112-128)
+    FIN_PCG_BETA = 14,  // beta = (r, z) / rho, rho = (r, z), check on (r, r)  (pcg.inl:130-152)
+    FIN_RB_ALPHA = 15,  // delta = (v, r0), alpha = rho / delta, -alpha        (rbicgstab.inl:104-115)
+    FIN_RB_OMEGA = 16   // omega, residual, check, rho, beta, -omega beta      (rbicgstab.inl:136-200)
 };
 
 template <typename F>
@@ -134,9 +141,11 @@ struct EwArgs {
     // AXPBYPCZ sets z = x (restart of a Krylov recurrence from its residual)
     const int *copy_if = nullptr;
     // AXPBYPCZ / MX_XUPD: leave z unchanged for an RHS whose coefficients a and b are
-    // both exactly 0 (the masked solution update of a converged RHS keeps its solution
-    // even when that RHS's directions have broken down to NaN)
+    // both exactly 0; AXPBY_Z: whose coefficient a is exactly 0 (the masked solution
+    // update of a converged RHS keeps its solution even when that RHS's directions have
+    // broken down to NaN)
     int freeze_zero = 0;
+    const T *w2 = nullptr; // EW_RB_R: third operand of the dots
 };
 
 struct LaunchCfg {

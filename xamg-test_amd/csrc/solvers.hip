@@ -1208,6 +1208,256 @@ class MultiGrid : public Solver<T> {
 };
 
 // ---------------------------------------------------------------------------
+// PCG (pcg.inl:47-157)
+// ---------------------------------------------------------------------------
+template <typename T>
+class PCG : public Solver<T> {
+  public:
+    std::unique_ptr<Solver<T>> precond;
+    std::shared_ptr<MgHierarchy> hier;
+
+    void setup() override {
+        Solver<T>::setup();
+        this->alloc_bufs(4);
+        XAMG_REQUIRE(this->gp->has("preconditioner"), "PCG needs a preconditioner block");
+        const ParamList &pp = this->gp->get("preconditioner");
+        precond = make_solver<T>(pp.get_string("method"), this->A, pp, "preconditioner", this->nv,
+                                 this->gp, this->level, hier);
+    }
+    void renew(const GlobalParams &g) override {
+        Solver<T>::renew(g);
+        precond->renew(g);
+    }
+    void apply(T *x, bool xzero, const T *b, const int *halt) override {
+        auto *A = this->A;
+        const int nv = this->nv;
+        auto *ks = this->ks;
+        auto *cs = this->cs;
+        state_init<T>(ks, cs, nv, this->hist_cap);
+        T *r = this->vec(0), *z = this->vec(1), *p = this->vec(2), *Ap = this->vec(3);
+        const int *H = this->conv_check ? &cs->halt : halt;
+        // r = b - A x, initial check on (r, r) (stats.h:208-219)
+        if (xzero)
+            fill<T>(x, A->n, nv, (T)0, halt);
+        resid_norm<T>(this, x, xzero, b, r, FIN_NORM_INIT, halt, true);
+        // z = M^{-1} r, p = z, rho = (r, z)
+        precond->apply(z, true, r, H);
+        {
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = p;
+            a.x = z;
+            ew<T>(EW_COPY3, nv, a, H);
+        }
+        {
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = p;
+            a.x = r;
+            a.w = z;
+            ew<T>(EW_DOTS, 3, nv, a, red_args(H), this->fin(FIN_PCG_RZ));
+        }
+        auto iteration = [&]() {
+            { // Ap = A p, alpha = rho / (p, Ap)
+                SpmvArgs<T> a{};
+                a.x = p;
+                a.y = Ap;
+                a.w = p;
+                spmv<T>(*A, this->vprec, EPI_AXY, 1, nv, a, red_args(H), this->fin(FIN_PCG_ALPHA));
+            }
+            { // x = alpha_c p + 1 x
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = x;
+                a.x = p;
+                a.a = ks->ac;
+                a.y = x;
+                a.freeze_zero = 1;
+                ew<T>(EW_AXPBY_Z, nv, a, H);
+            }
+            { // r = -alpha Ap + 1 r
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = r;
+                a.x = Ap;
+                a.a = ks->nalpha;
+                a.y = r;
+                ew<T>(EW_AXPBY_Z, nv, a, H);
+            }
+            precond->apply(z, true, r, H);
+            { // (r, z), (r, r): beta, convergence check
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = p;
+                a.x = r;
+                a.w = z;
+                ew<T>(EW_DOTS, 3, nv, a, red_args(H), this->fin(FIN_PCG_BETA, true));
+            }
+            { // p = 1 z + beta p
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = p;
+                a.x = z;
+                a.y = p;
+                a.b = ks->beta;
+                RedArgs ra = red_args(H);
+                ra.set_halt_if_done = this->conv_check;
+                ra.cs = cs;
+                ew<T>(EW_AXPBY_Z, 0, nv, a, ra, FinArgs<T>{});
+            }
+        };
+        if (this->conv_check)
+            run_loop(this->max_iters, cs, iteration);
+        else
+            for (int it = 0; it < this->max_iters; it++)
+                iteration();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// RBiCGStab (rbicgstab.inl:47-205): right-preconditioned BiCGStab that carries M^{-1} p
+// as a recurrence and gathers all four dots of an iteration in one reduction
+// ---------------------------------------------------------------------------
+template <typename T>
+class RBiCGStab : public Solver<T> {
+  public:
+    std::unique_ptr<Solver<T>> precond;
+    std::shared_ptr<MgHierarchy> hier;
+
+    void setup() override {
+        Solver<T>::setup();
+        this->alloc_bufs(8);
+        XAMG_REQUIRE(this->gp->has("preconditioner"), "RBiCGStab needs a preconditioner block");
+        const ParamList &pp = this->gp->get("preconditioner");
+        precond = make_solver<T>(pp.get_string("method"), this->A, pp, "preconditioner", this->nv,
+                                 this->gp, this->level, hier);
+    }
+    void renew(const GlobalParams &g) override {
+        Solver<T>::renew(g);
+        precond->renew(g);
+    }
+    void apply(T *x, bool xzero, const T *b, const int *halt) override {
+        auto *A = this->A;
+        const int nv = this->nv;
+        auto *ks = this->ks;
+        auto *cs = this->cs;
+        state_init<T>(ks, cs, nv, this->hist_cap);
+        T *r0 = this->vec(0), *r = this->vec(1), *z = this->vec(2), *v_ = this->vec(3),
+          *v = this->vec(4), *s = this->vec(5), *t_ = this->vec(6), *t = this->vec(7);
+        const int *H = this->conv_check ? &cs->halt : halt;
+        // 1-2: r0 = b - A x0, rho = (r0, r0), r = r0
+        if (xzero) {
+            fill<T>(x, A->n, nv, (T)0, halt);
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = r0;
+            a.x = b;
+            a.y2 = r;
+            ew<T>(EW_COPY3, 1, nv, a, red_args(halt), this->fin(FIN_INIT, true));
+        } else {
+            SpmvArgs<T> a{};
+            a.x = x;
+            a.y = r0;
+            a.b = b;
+            a.y2 = r;
+            spmv<T>(*A, this->vprec, EPI_RESID, 1, nv, a, red_args(halt), this->fin(FIN_INIT, true));
+        }
+        precond->apply(z, true, r0, H); // 3
+        {                               // 4: v_ = z
+            EwArgs<T> a;
+            a.n = A->n;
+            a.z = v_;
+            a.x = z;
+            ew<T>(EW_COPY3, nv, a, H);
+        }
+        auto iteration = [&]() {
+            { // 6-7, 9: v = A v_, delta = (v, r0), alpha = rho / delta
+                SpmvArgs<T> a{};
+                a.x = v_;
+                a.y = v;
+                a.w = r0;
+                spmv<T>(*A, this->vprec, EPI_AXY, 1, nv, a, red_args(H), this->fin(FIN_RB_ALPHA));
+            }
+            precond->apply(s, true, v, H); // 8
+            {                              // 10: t_ = 1 z - alpha s
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = t_;
+                a.x = z;
+                a.y = s;
+                a.b = ks->nalpha;
+                ew<T>(EW_AXPBY_Z, nv, a, H);
+            }
+            { // 11: t = A t_
+                SpmvArgs<T> a{};
+                a.x = t_;
+                a.y = t;
+                spmv<T>(*A, this->vprec, EPI_AXY, 0, nv, a, red_args(H), FinArgs<T>{});
+            }
+            { // 12-13, 15, 17, 21-22: r = -alpha v + r; theta, phi, psi, eta; omega, check
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = r;
+                a.x = v;
+                a.a = ks->nalpha;
+                a.y = r;
+                a.w = t;
+                a.w2 = r0;
+                ew<T>(EW_RB_R, 4, nv, a, red_args(H), this->fin(FIN_RB_OMEGA, true));
+            }
+            precond->apply(z, true, t, H); // 14
+            {                              // 16: r = -omega t + r
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = r;
+                a.x = t;
+                a.a = ks->nomega;
+                a.y = r;
+                ew<T>(EW_AXPBY_Z, nv, a, H);
+            }
+            { // 23: x = alpha_c v_ + omega_c t_ + x
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = x;
+                a.x = v_;
+                a.a = ks->ac;
+                a.y = t_;
+                a.b = ks->oc;
+                a.freeze_zero = 1;
+                ew<T>(EW_AXPBYPCZ, nv, a, H);
+            }
+            { // 24: z = 1 t_ - omega z (the loop ends here once converged)
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = z;
+                a.x = t_;
+                a.y = z;
+                a.b = ks->nomega;
+                RedArgs ra = red_args(H);
+                ra.set_halt_if_done = this->conv_check;
+                ra.cs = cs;
+                ew<T>(EW_AXPBY_Z, 0, nv, a, ra, FinArgs<T>{});
+            }
+            { // 25: v_ = 1 z - beta omega s + beta v_
+                EwArgs<T> a;
+                a.n = A->n;
+                a.z = v_;
+                a.x = z;
+                a.y = s;
+                a.b = ks->nbo;
+                a.c = ks->beta;
+                ew<T>(EW_AXPBYPCZ, nv, a, H);
+            }
+        };
+        if (this->conv_check)
+            run_loop(this->max_iters, cs, iteration);
+        else
+            for (int it = 0; it < this->max_iters; it++)
+                iteration();
+    }
+};
+
+// ---------------------------------------------------------------------------
 // factory (solver.h:352-399)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1230,6 +1480,14 @@ std::unique_ptr<Solver<T>> make_solver(const std::string &method, DevMatrix *A,
              method == "MergedPBiCGStab") {
         auto *k = new Krylov<T>();
         k->pre = (method == "PBiCGStab" || method == "MergedPBiCGStab");
+        k->hier = hier;
+        s.reset(k);
+    } else if (method == "PCG") {
+        auto *k = new PCG<T>();
+        k->hier = hier;
+        s.reset(k);
+    } else if (method == "RBiCGStab" || method == "MergedRBiCGStab") {
+        auto *k = new RBiCGStab<T>();
         k->hier = hier;
         s.reset(k);
     } else if (method == "MultiGrid") {
