@@ -11,13 +11,16 @@ Chebyshev, FP64 solution and residual replacement), inputs resident in HBM.  The
 FP64 GPU solve of the same system is timed the same way; `value` is the mixed-precision
 time to tolerance in seconds (lower is better) and the line also carries the FP64 time,
 the speedup, the final FP64-verified residuals and the error against the FP64 solution,
-the SpMV bandwidth with its roofline record, the CPU baseline (the oracle's OpenMP
+the per-launch device times of the mixed solve (HIP events around every dispatch) with
+the roofline record of its dominant kernel, the standalone SpMV and copy bandwidths, the
+CPU baseline (the oracle's OpenMP
 restatement of the reference path, timed on a bounded sample of iterations on the host
 cores) and time-to-tolerance records of the other workloads (other_configs).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -341,6 +344,14 @@ def main():
         r = (b - ax).reshape(n, nv)
         return (np.linalg.norm(r, axis=0) / np.linalg.norm(b.reshape(n, nv), axis=0)).max()
 
+    # Live device time per launch class of the same mixed solve, re-run `steps` times with
+    # HIP events recorded on the launch stream around every dispatch
+    X.kernel_timing(True)
+    t_inst = run(Smx, args.steps)
+    X.kernel_timing(False)
+    kstats = X.kernel_stats()
+    ktot = sum(k["ms"] for k in kstats)
+
     rr64, rrmx = relres(x64), relres(xmx)
     err_mx = np.linalg.norm(xmx - x64) / np.linalg.norm(x64)
 
@@ -354,12 +365,21 @@ def main():
     gbs64 = b64 / (ms64 * 1e-3) / 1e9
     copy_gbs = X.bench_copy(2 << 30, iters=20)
 
-    # HBM bytes per launch of the same kernel and workload from the committed rocprofv3
-    # counter passes (FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated)
+    # The dominant kernel of the mixed solve (largest total device time) and its HBM bytes
+    # per launch from the committed rocprofv3 counter passes over the same solve
+    # (FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated)
+    dom = kstats[0]
+    dom_bytes = dom["bytes"] / dom["calls"]
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"spmv_7pt{n1}_nv{nv}.json")
+    prof = os.path.join(ROOT, "profiles", f"solve_7pt{n1}_nv{nv}_{args.solver}.json")
     if os.path.exists(prof):
-        traffic = json.load(open(prof))["fp32"]["traffic_bytes"]
+        cls = json.load(open(prof)).get("classes", {})
+        rec = cls.get(dom["name"])
+        if rec is None:  # same kernel under another RHS-per-lane choice of the autotuner
+            base = re.sub(r" vpt\d+", "", dom["name"])
+            rec = next((v for k, v in cls.items() if re.sub(r" vpt\d+", "", k) == base), None)
+        if rec is not None:
+            traffic = rec["traffic_bytes_per_launch"]
 
     mean_mx = float(np.mean(tmx))
     mean_64 = float(np.mean(t64))
@@ -392,10 +412,17 @@ def main():
         "spmv": {"fp32_gbs": gbs32, "fp64_gbs": gbs64, "fp32_ms": ms32, "fp64_ms": ms64,
                  "copy_gbs": copy_gbs, "fp32_frac_of_copy": gbs32 / copy_gbs,
                  "fp64_frac_of_copy": gbs64 / copy_gbs},
-        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_spmv<float,float,8,EPI_AXY,0>",
-                     "algorithmic_bytes": b32},
+        "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": dom["name"], "algorithmic_bytes": dom_bytes,
+                     "avg_launch_us": dom["avg_us"], "launches": dom["calls"],
+                     "share_of_device_time": dom["ms"] / ktot,
+                     "timing": "HIP events on the launch stream around every dispatch of "
+                               f"{args.steps} mixed solves"},
+        "kernels": [{"name": k["name"], "launches": k["calls"], "avg_us": round(k["avg_us"], 2),
+                     "gbs": round(k["gbs"], 1), "share": round(k["ms"] / ktot, 4)}
+                    for k in kstats[:10]],
+        "instrumented_solve_s": float(np.mean(t_inst)),
         "device": dev,
     }
     if not args.no_cpu:

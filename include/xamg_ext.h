@@ -53,6 +53,26 @@ int xamgx_bench_spmv(XAMG_matrix_id mid, int prec, int nv, int iters, double *av
 /* Sustained copy bandwidth (GB/s, read+write bytes) of a streaming copy kernel over
  * `bytes` bytes, best of `iters` launches. */
 int xamgx_bench_copy(uint64_t bytes, int iters, double *gbs);
+/* Average device time (ms) of one launch of an SpMV epilogue (0 y=Ax, 1 y+=Ax, 2 residual,
+ * 3 Jacobi step, 4 Chebyshev step, 5 last Chebyshev step, 6 FP64 residual stored as FP32)
+ * with nd fused dot products (0, 1 or 3 as the solvers use them) on operand buffers of
+ * the matrix's shape.  prec 0 FP64, 1 FP32, 2 BF16 values with FP32 vectors, 3 scaled
+ * FP32.  cfg = {RHS per lane, workgroups, XCD mapping, LDS staging} or NULL for the
+ * autotuned choice.  cold = 1 times each launch alone after a 512 MiB copy has evicted
+ * the caches (as inside a solve); 0 times iters launches back to back. */
+int xamgx_bench_spmv_epi(XAMG_matrix_id mid, int prec, int nv, int epi, int nd, int cold, int iters,
+                         const int *cfg, double *avg_ms);
+/* Per-launch device timing of the solve path: HIP events on the launch stream around
+ * every SpMV and element-wise dispatch (a dispatch with fused dot products includes its
+ * finish kernel), grouped by launch class with the algorithmic HBM bytes of each launch.
+ * on = 1 clears the record and starts recording, 0 stops.  Launches captured into a
+ * graph are not recorded.  Has no counterpart in the reference. */
+int xamgx_kernel_timing(int on);
+/* Number of launch classes recorded (waits for the device). */
+int xamgx_kernel_stats_count(int *count);
+/* Class i: name (NUL-terminated, at most len bytes), launches, total device ms, total
+ * algorithmic bytes.  NULL skips an output. */
+int xamgx_kernel_stats_get(int i, char *name, int len, int64_t *calls, double *ms, double *bytes);
 /* Device time (ms) of the last solve (first kernel to the convergence readback). */
 int xamgx_solver_last_ms(XAMG_solver_id sid, double *ms);
 
@@ -72,6 +92,9 @@ int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int 
 /* Right-hand sides per lane of the SpMV kernel for nv and element size es (8: FP64
  * vectors, 4: FP32 vectors); 0 restores the default.  Used by the autotuner. */
 int xamgx_set_spmv_vpt(int nv, int es, int vpt);
+/* LDS staging of each row tile's column indices and values in every SpMV launch:
+ * -1 as autotuned (default), 0 off, 1 on.  Results are identical either way. */
+int xamgx_set_spmv_stage(int mode);
 /* Spectrum bounds of D^{-1/2} A D^{-1/2} used by Chebyshev (the reference takes
  * them from hypre's setup, hypre_base.inl:369-375). */
 int xamgx_matrix_set_eigs(XAMG_matrix_id mid, double lo, double hi);

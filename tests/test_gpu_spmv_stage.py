@@ -1,0 +1,112 @@
+"""The LDS-staged SpMV (column indices and values of each row tile copied to LDS first)
+against the oracle, bit for bit, at every compiled RHS-per-lane width; tiles whose
+nonzeros exceed the LDS capacity take the direct path inside the same launch; a solve
+gives identical iterates with staging forced on and off."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import bf16_round
+
+pytestmark = pytest.mark.gpu
+
+VPTS = {1: [1], 2: [1, 2], 4: [2, 4], 8: [2, 4, 8], 16: [4, 8, 16]}
+
+
+@pytest.fixture
+def staged(xamg):
+    xamg.set_spmv_stage(1)
+    yield xamg
+    xamg.set_spmv_stage(-1)
+    for nv, vs in VPTS.items():
+        for es in (4, 8):
+            xamg.set_spmv_vpt(nv, es, 0)
+
+
+@pytest.mark.parametrize("nv", [1, 2, 4, 8, 16])
+def test_staged_fp64_fp32_bitexact(staged, nv):
+    X = staged
+    row, col, val = X.poisson7(24)
+    A = X.Matrix(row, col, val)
+    n = A.n
+    x = X.rhs(n, nv, 7)
+    ref64 = O.spmv(O.csr(row, col, val), x, nv)
+    x32 = x.astype(np.float32)
+    ref32 = O.spmv(O.csr(row, col, val.astype(np.float32)), x32, nv)
+    y0 = X.rhs(n, nv, 4)
+    refax = O.spmv(O.csr(row, col, val), x, nv, axpy_into=y0.copy())
+    for vpt in VPTS[nv]:
+        print(f"stage=1 nv={nv} vpt={vpt}", flush=True)
+        X.set_spmv_vpt(nv, 8, vpt)
+        X.set_spmv_vpt(nv, 4, vpt)
+        y = A.spmv(x, nv, prec=0)
+        assert np.array_equal(y.view(np.uint64), ref64.view(np.uint64)), vpt
+        y = A.spmv(x32, nv, prec=1)
+        assert np.array_equal(y.view(np.uint32), ref32.view(np.uint32)), vpt
+        y = A.spmv(x, nv, prec=0, y=y0, mode=1)
+        assert np.array_equal(y.view(np.uint64), refax.view(np.uint64)), vpt
+    A.destroy()
+
+
+@pytest.mark.parametrize("nv", [1, 8])
+def test_staged_bf16_27point(staged, nv):
+    # 27 nonzeros per row: tiles of 128 rows exceed the LDS capacity (direct path),
+    # tiles of 64 rows fit
+    X = staged
+    row, col, val = X.aniso27(12, 10, 9)
+    A = X.Matrix(row, col, val)
+    x = X.rhs(A.n, nv, 9).astype(np.float32)
+    ref = O.spmv(O.csr(row, col, bf16_round(val)), x, nv)
+    for vpt in VPTS[nv]:
+        X.set_spmv_vpt(nv, 4, vpt)
+        y = A.spmv(x, nv, prec=2)
+        assert np.array_equal(y.view(np.uint32), ref.view(np.uint32)), vpt
+    A.destroy()
+
+
+def test_staged_ragged_long_and_empty_rows(staged):
+    # rows of 0..9 nonzeros with a few rows of 1500-3000: some tiles exceed the LDS
+    # capacity and are read directly, their neighbours are staged
+    X = staged
+    rng = np.random.default_rng(12)
+    n = 6000
+    lens = rng.integers(0, 10, n)
+    lens[::17] = 0
+    lens[[100, 2500, 2501, 5999]] = [3000, 1500, 2100, 2048]
+    row = np.zeros(n + 1, dtype=np.int64)
+    row[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    val = rng.standard_normal(len(col))
+    A = X.Matrix(row.astype(np.int32), col, val)
+    for nv in (1, 4, 8):
+        x = rng.standard_normal(n * nv)
+        ref = O.spmv(O.csr(row, col, val), x, nv)
+        for vpt in VPTS[nv]:
+            X.set_spmv_vpt(nv, 8, vpt)
+            y = A.spmv(x, nv)
+            assert np.array_equal(y.view(np.uint64), ref.view(np.uint64)), (nv, vpt)
+    A.destroy()
+
+
+def test_staged_solve_identical_iterates(xamg):
+    # PBiCGStab + Chebyshev(4): every fused epilogue and dot path of the loop
+    m, nv = 20, 4
+    row, col, val = xamg.poisson7(m)
+    n = m ** 3
+    c = np.cos(np.pi / (m + 1))
+    b = xamg.rhs(n, nv, 5)
+    params = {"solver": {"method": "PBiCGStab", "rel_tolerance": 1e-10, "abs_tolerance": 0.0,
+                         "max_iters": 200},
+              "preconditioner": {"method": "Chebyshev", "polynomial_order": 4,
+                                 "spectrum_fraction": 0.3}}
+    outs = []
+    try:
+        for mode in (0, 1):
+            xamg.set_spmv_stage(mode)
+            outs.append(xamg.solve_system(row, col, val, b, nv, params, eigs=(1 - c, 1 + c)))
+    finally:
+        xamg.set_spmv_stage(-1)
+    a, s = outs
+    assert a["iters"] == s["iters"]
+    assert np.array_equal(a["x"].view(np.uint64), s["x"].view(np.uint64))
+    assert np.array_equal(a["hist"], s["hist"])

@@ -1,14 +1,20 @@
-// SpMV autotuner: for each (matrix, value precision, vector precision, nv) it times the
-// compiled RHS-per-lane widths (which fix rows per wavefront = 64 * VPT / nv and the
-// nonzero unroll 16 / VPT), the workgroup counts and the XCD-aware row mapping on the
-// matrix itself, and keeps the fastest.  Results are cached per process under a key of
-// the matrix shape and optionally in a text file (XAMG_TUNE_FILE) across processes.
+// SpMV autotuner: for each (matrix, value precision, vector precision, nv, epilogue,
+// fused dots) it times the compiled RHS-per-lane widths (which fix rows per wavefront =
+// 64 * VPT / nv and the nonzero unroll 16 / VPT), the workgroup counts, the XCD-aware row
+// mapping and the LDS staging of column indices and values on the matrix itself, and
+// keeps the fastest.  Each candidate launch is timed cold: a 512 MiB copy before it
+// evicts the L2 caches and the 256 MB last-level cache, as the other vectors of a
+// solver iteration do between two SpMVs (timed back to back, the matrix and vectors of a
+// 128^3 system largely stay in the last-level cache and the ranking changes).  Results
+// are cached per process under a key of the matrix shape and optionally in a text file
+// (XAMG_TUNE_FILE) across processes.
 #include "runtime.hpp"
 
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 
@@ -31,7 +37,7 @@ std::map<std::string, DevMatrix::Tune> &cache() {
             std::ifstream in(f);
             std::string key;
             DevMatrix::Tune t;
-            while (in >> key >> t.vpt >> t.grid >> t.remap) {
+            while (in >> key >> t.vpt >> t.grid >> t.remap >> t.mode) {
                 t.valid = true;
                 c[key] = t;
             }
@@ -45,7 +51,7 @@ void save(const std::string &key, const DevMatrix::Tune &t) {
     if (!f)
         return;
     std::ofstream out(f, std::ios::app);
-    out << key << " " << t.vpt << " " << t.grid << " " << t.remap << "\n";
+    out << key << " " << t.vpt << " " << t.grid << " " << t.remap << " " << t.mode << "\n";
 }
 
 std::vector<int> vpt_options(int nv) {
@@ -102,47 +108,72 @@ struct TuneBufs {
     }
 };
 
+} // namespace
+
+CacheFlush::CacheFlush() : src(kBytes), dst(kBytes) {} // contents irrelevant
+
+void CacheFlush::run() { launch_copy(src.p, dst.p, kBytes, Runtime::get().stream); }
+
 template <typename T>
-float time_cfg(DevMatrix &A, Prec vp, int nv, int epi, int nd, SpmvArgs<T> a, int vpt, int grid,
-               int remap) {
+float time_spmv_cfg(DevMatrix &A, Prec vp, int nv, int epi, int nd, SpmvArgs<T> a, int reps,
+                    CacheFlush *flush) {
     hipStream_t s = Runtime::get().stream;
-    a.cfg_vpt = vpt;
-    a.cfg_grid = grid;
-    a.cfg_remap = remap;
     hipEvent_t e0, e1;
     XAMG_HIP_CHECK(hipEventCreate(&e0));
     XAMG_HIP_CHECK(hipEventCreate(&e1));
     spmv<T>(A, vp, epi, nd, nv, a, red_args(), FinArgs<T>{}); // warm-up
-    XAMG_HIP_CHECK(hipEventRecord(e0, s));
-    const int reps = 4;
-    for (int i = 0; i < reps; i++)
-        spmv<T>(A, vp, epi, nd, nv, a, red_args(), FinArgs<T>{});
-    XAMG_HIP_CHECK(hipEventRecord(e1, s));
-    XAMG_HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0;
-    XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    float tot = 0;
+    if (flush != nullptr) {
+        for (int i = 0; i < reps; i++) {
+            flush->run();
+            XAMG_HIP_CHECK(hipEventRecord(e0, s));
+            spmv<T>(A, vp, epi, nd, nv, a, red_args(), FinArgs<T>{});
+            XAMG_HIP_CHECK(hipEventRecord(e1, s));
+            XAMG_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            tot += ms;
+        }
+    } else {
+        XAMG_HIP_CHECK(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; i++)
+            spmv<T>(A, vp, epi, nd, nv, a, red_args(), FinArgs<T>{});
+        XAMG_HIP_CHECK(hipEventRecord(e1, s));
+        XAMG_HIP_CHECK(hipEventSynchronize(e1));
+        XAMG_HIP_CHECK(hipEventElapsedTime(&tot, e0, e1));
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return ms / reps;
+    return tot / reps;
 }
+
+namespace {
 
 template <typename T>
 DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
+    KernelTimer::Pause quiet; // tuning launches are not part of any timed solve
     TuneBufs<T> bufs(A, nv);
-    const SpmvArgs<T> a = bufs.args(epi);
+    CacheFlush flush;
+    SpmvArgs<T> a = bufs.args(epi);
     DevMatrix::Tune best;
     float best_ms = 1e30f;
-    for (int vpt : vpt_options(nv))
-        for (int grid : {1024, 2048, 4096, 8192})
-            for (int remap : {0, 1}) {
-                float ms = time_cfg<T>(A, vp, nv, epi, nd, a, vpt, grid, remap);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best.vpt = vpt;
-                    best.grid = grid;
-                    best.remap = remap;
+    for (int mode : {0, 1})
+        for (int vpt : vpt_options(nv))
+            for (int grid : {1024, 2048, 4096, 8192})
+                for (int remap : {0, 1}) {
+                    a.cfg_vpt = vpt;
+                    a.cfg_grid = grid;
+                    a.cfg_remap = remap;
+                    a.cfg_mode = mode;
+                    const float ms = time_spmv_cfg<T>(A, vp, nv, epi, nd, a, 3, &flush);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best.vpt = vpt;
+                        best.grid = grid;
+                        best.remap = remap;
+                        best.mode = mode;
+                    }
                 }
-            }
     best.valid = true;
     return best;
 }
@@ -159,8 +190,8 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
     if (!autotune_enabled() || n < 65536)
         return t;
     std::ostringstream k;
-    k << "n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv" << nv
-      << "_epi" << epi << "_nd" << nd;
+    k << "cold2_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
+      << nv << "_epi" << epi << "_nd" << nd;
     const std::string key = k.str();
     auto &c = cache();
     auto it = c.find(key);
@@ -176,4 +207,35 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
     return tune[slot];
 }
 
+template <typename T>
+static float bench_epi(DevMatrix &A, Prec vp, int nv, int epi, int nd, int cold, int iters,
+                       const int *cfg) {
+    KernelTimer::Pause quiet;
+    TuneBufs<T> bufs(A, nv);
+    SpmvArgs<T> a = bufs.args(epi);
+    if (cfg != nullptr && cfg[0] > 0) {
+        a.cfg_vpt = cfg[0];
+        a.cfg_grid = cfg[1];
+        a.cfg_remap = cfg[2];
+        a.cfg_mode = cfg[3];
+    }
+    std::unique_ptr<CacheFlush> flush;
+    if (cold)
+        flush.reset(new CacheFlush());
+    return time_spmv_cfg<T>(A, vp, nv, epi, nd, a, iters, flush.get());
+}
+
+float bench_spmv_epi(DevMatrix &A, Prec vp, int nv, int epi, int nd, int cold, int iters,
+                     const int *cfg) {
+    return vp == Prec::F64 ? bench_epi<double>(A, vp, nv, epi, nd, cold, iters, cfg)
+                           : bench_epi<float>(A, vp, nv, epi, nd, cold, iters, cfg);
+}
+
+} // namespace xamg
+
+namespace xamg {
+template float time_spmv_cfg<double>(DevMatrix &, Prec, int, int, int, SpmvArgs<double>, int,
+                                     CacheFlush *);
+template float time_spmv_cfg<float>(DevMatrix &, Prec, int, int, int, SpmvArgs<float>, int,
+                                    CacheFlush *);
 } // namespace xamg

@@ -881,6 +881,58 @@ int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int 
     });
 }
 
+int xamgx_bench_spmv_epi(XAMG_matrix_id mid, int prec, int nv, int epi, int nd, int cold, int iters,
+                         const int *cfg, double *avg_ms) {
+    return guard([&] {
+        check_nv(nv);
+        DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
+        XAMG_REQUIRE(A != nullptr, "matrix not constructed");
+        XAMG_REQUIRE(prec >= 0 && prec <= 3, "prec must be 0 (FP64), 1 (FP32), 2 (BF16) or 3 (scaled FP32)");
+        XAMG_REQUIRE(epi >= EPI_AXY && epi <= EPI_RESID_MX, "unknown epilogue");
+        XAMG_REQUIRE(epi != EPI_RESID_MX || prec == 0, "the FP32-output residual needs FP64");
+        XAMG_REQUIRE(iters > 0, "iters must be positive");
+        const Prec vp = prec == 0 ? Prec::F64 : prec == 1 ? Prec::F32 : prec == 2 ? Prec::BF16 : Prec::F32S;
+        *avg_ms = bench_spmv_epi(*A, vp, nv, epi, nd, cold, iters, cfg);
+    });
+}
+
+int xamgx_kernel_timing(int on) {
+    return guard([&] {
+        if (on)
+            KernelTimer::get().start();
+        else
+            KernelTimer::get().stop();
+    });
+}
+
+int xamgx_kernel_stats_count(int *count) {
+    return guard([&] { *count = (int)KernelTimer::get().stats().size(); });
+}
+
+int xamgx_kernel_stats_get(int i, char *name, int len, int64_t *calls, double *ms, double *bytes) {
+    return guard([&] {
+        const auto &st = KernelTimer::get().stats();
+        XAMG_REQUIRE(i >= 0 && i < (int)st.size(), "kernel class index out of range");
+        if (name != nullptr && len > 0) {
+            std::strncpy(name, st[i].name.c_str(), (size_t)len - 1);
+            name[len - 1] = 0;
+        }
+        if (calls)
+            *calls = st[i].calls;
+        if (ms)
+            *ms = st[i].ms;
+        if (bytes)
+            *bytes = st[i].bytes;
+    });
+}
+
+int xamgx_set_spmv_stage(int mode) {
+    return guard([&] {
+        XAMG_REQUIRE(mode >= -1 && mode <= 1, "mode must be -1 (tuned), 0 or 1");
+        launch_cfg().spmv_stage = mode;
+    });
+}
+
 int xamgx_set_spmv_vpt(int nv, int es, int vpt) {
     return guard([&] {
         check_nv(nv);

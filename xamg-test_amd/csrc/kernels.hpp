@@ -119,8 +119,9 @@ struct SpmvArgs {
     T *y2 = nullptr, *y3 = nullptr; // extra copies of y (initial residual: r, p)
     // EPI_RESID_MX: FP32 copies of the FP64 residual (y is not written)
     float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr;
-    // launch configuration chosen by the autotuner (0 / -1: global defaults)
-    int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1;
+    // launch configuration chosen by the autotuner (0 / -1: global defaults); cfg_mode 1
+    // stages each row tile's column indices and values through LDS
+    int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1, cfg_mode = 0;
 };
 
 template <typename T>
@@ -153,6 +154,7 @@ struct LaunchCfg {
     int grid_ew = 2048;
     int grid_red = 2048;
     int xcd_remap = 1;
+    int spmv_stage = -1; // SpMV LDS staging: -1 as tuned, 0 off, 1 on for every launch
 };
 
 LaunchCfg &launch_cfg();
@@ -170,6 +172,16 @@ void launch_spmv(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs
 template <typename T>
 void launch_ew(int op, int ndots, int nv, const EwArgs<T> &a, const RedArgs &r,
                const FinArgs<T> &f, hipStream_t s);
+
+// The LDS-staged SpMV instantiations (spmv_staged.hip).
+template <typename V, typename T>
+void launch_spmv_staged(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                        const FinArgs<T> &f, hipStream_t s);
+
+// Final pass of a fused reduction over G workgroup partials (nd dot products per RHS),
+// applying the solver's scalar update.
+template <typename T, int NV>
+void launch_finish(int nd, const RedArgs &r, const FinArgs<T> &f, int G, hipStream_t s);
 
 // Dense GEMV y = A x for the coarse-grid Direct solver (dense_matvec.inl:100-117);
 // `at` holds A column-major.

@@ -209,6 +209,131 @@ void DevMatrix::estimate_eigs(int steps) {
     amg::lanczos_bounds(n, hrow.data(), hcol.data(), hval.data(), steps, &min_eig, &max_eig);
 }
 
+KernelTimer &KernelTimer::get() {
+    static KernelTimer kt;
+    return kt;
+}
+
+void KernelTimer::start() {
+    stats();
+    st_.clear();
+    idx_.clear();
+    on_ = true;
+}
+
+hipEvent_t KernelTimer::take() {
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    XAMG_HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+int KernelTimer::begin(const std::string &cls, double bytes) {
+    if (!on_)
+        return -1;
+    hipStream_t s = Runtime::get().stream;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    XAMG_HIP_CHECK(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return -1;
+    auto it = idx_.find(cls);
+    int k;
+    if (it == idx_.end()) {
+        k = (int)st_.size();
+        idx_[cls] = k;
+        st_.push_back(Stat{cls});
+    } else {
+        k = it->second;
+    }
+    Pending p{k, bytes, take(), take()};
+    XAMG_HIP_CHECK(hipEventRecord(p.e0, s));
+    pend_.push_back(p);
+    return (int)pend_.size() - 1;
+}
+
+void KernelTimer::end(int token) {
+    if (token < 0)
+        return;
+    XAMG_HIP_CHECK(hipEventRecord(pend_[token].e1, Runtime::get().stream));
+}
+
+const std::vector<KernelTimer::Stat> &KernelTimer::stats() {
+    if (!pend_.empty()) {
+        Runtime::get().sync();
+        for (Pending &p : pend_) {
+            float ms = 0;
+            XAMG_HIP_CHECK(hipEventElapsedTime(&ms, p.e0, p.e1));
+            Stat &s = st_[p.stat];
+            s.calls++;
+            s.ms += ms;
+            s.bytes += p.bytes;
+            pool_.push_back(p.e0);
+            pool_.push_back(p.e1);
+        }
+        pend_.clear();
+    }
+    return st_;
+}
+
+static const char *prec_tag(Prec p) {
+    switch (p) {
+    case Prec::F64: return "f64";
+    case Prec::F32: return "f32";
+    case Prec::BF16: return "bf16";
+    default: return "f32s";
+    }
+}
+
+static const char *epi_tag(int epi) {
+    static const char *t[] = {"axy", "axpy", "resid", "jacobi", "cheb", "cheb_last", "resid_mx"};
+    return epi >= 0 && epi < 7 ? t[epi] : "?";
+}
+
+// Algorithmic HBM bytes of one SpMV launch: row pointers, column indices and values
+// once, x once (ncols rows), every row-wise operand the epilogue reads, every output.
+template <typename T>
+static double spmv_launch_bytes(const DevMatrix &A, Prec vp, int epi, int nd, int nv,
+                                const SpmvArgs<T> &a) {
+    const double sv = vp == Prec::F64 ? 8 : vp == Prec::BF16 ? 2 : 4, st = sizeof(T);
+    const double E = (double)A.n * nv * st, Dv = (double)A.n * st;
+    double b = 4.0 * (A.n + 1) + A.nnz * (4.0 + sv) + (double)A.ncols * nv * st;
+    if (epi == EPI_RESID_MX)
+        b += (double)A.n * nv * 4 * (1 + (a.yo2 != nullptr) + (a.yo3 != nullptr));
+    else
+        b += E * (1 + (a.y2 != nullptr) + (a.y3 != nullptr));
+    if (epi == EPI_AXPY)
+        b += E;
+    if (epi != EPI_AXY && epi != EPI_AXPY && (const void *)a.b != (const void *)a.x)
+        b += E; // b (the first Chebyshev step gathers b itself as x)
+    if (epi == EPI_JACOBI)
+        b += E + Dv; // previous iterate, inverse diagonal
+    if (epi == EPI_CHEB || epi == EPI_CHEB_LAST) {
+        b += Dv;
+        if (a.dcol != nullptr && a.dcol != a.d)
+            b += Dv; // column scaling held apart from the row scaling
+        if (epi == EPI_CHEB_LAST && a.xin != nullptr)
+            b += E;
+    }
+    if (nd > 0 && a.w != nullptr)
+        b += E;
+    return b;
+}
+
+// RHS per lane the SpMV launcher uses for a requested width (its fallbacks included).
+static int eff_vpt(int nv, int req) {
+    switch (nv) {
+    case 1: return 1;
+    case 2: return req == 1 ? 1 : 2;
+    case 4: return req == 2 ? 2 : 4;
+    case 8: return req == 2 || req == 8 ? req : 4;
+    default: return req == 8 || req == 16 ? req : 4;
+    }
+}
+
 template <typename T>
 void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const RedArgs &r,
           const FinArgs<T> &f) {
@@ -223,6 +348,20 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
         a.cfg_vpt = t.vpt;
         a.cfg_grid = t.grid;
         a.cfg_remap = t.remap;
+        a.cfg_mode = t.mode;
+    }
+    if (launch_cfg().spmv_stage >= 0)
+        a.cfg_mode = launch_cfg().spmv_stage;
+    KernelTimer &kt = KernelTimer::get();
+    int tok = -1;
+    if (kt.on()) {
+        char cls[160];
+        // one class per compiled kernel and matrix (k_spmv<V, T, nv, vpt, epilogue, dots>)
+        snprintf(cls, sizeof(cls), "spmv %s/%s %s nd%d nv%d stage%d vpt%d n%lld", prec_tag(vp),
+                 sizeof(T) == 8 ? "f64" : "f32", epi_tag(epi), nd, nv, a.cfg_mode == 1 ? 1 : 0,
+                 eff_vpt(nv, a.cfg_vpt > 0 ? a.cfg_vpt : spmv_vpt_choice(nv, (int)sizeof(T))),
+                 (long long)A.n);
+        tok = kt.begin(cls, spmv_launch_bytes<T>(A, vp, epi, nd, nv, a));
     }
     if constexpr (std::is_same<T, double>::value) {
         XAMG_REQUIRE(vp == Prec::F64, "FP64 vectors need FP64 matrix values");
@@ -241,11 +380,48 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
             launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
         }
     }
+    kt.end(tok);
+}
+
+// Algorithmic HBM bytes of one element-wise launch (each operand once; data-dependent
+// branches such as frozen or copied right-hand sides are counted as the common case).
+template <typename T>
+static double ew_launch_bytes(int op, int nd, int nv, const EwArgs<T> &a) {
+    const double st = sizeof(T), so = sizeof(T) == 8 ? 4 : 8;
+    const double E = (double)a.n * nv * st, Eo = (double)a.n * nv * so, Dv = (double)a.n * st;
+    double b = op == EW_DOTS ? 0 : E; // z written
+    switch (op) {
+    case EW_AXPBY_Z: b += 2 * E; break;
+    case EW_RB_R: b += 4 * E; break; // x, y, w, w2
+    case EW_AXPBYPCZ: b += 3 * E; break;
+    case EW_JACOBI0: case EW_CHEB0: b += E + Dv; break;
+    case EW_CHEB0_LAST: b += E + Dv + (a.y != nullptr ? E : 0); break;
+    case EW_COPY3: b += E * (1 + (a.y2 != nullptr) + (a.y3 != nullptr)); break;
+    case EW_ADD_CONV: b += Eo + E; break;
+    case EW_CONVERT: b += Eo; break;
+    case EW_SCAL_D: b += E + Dv; break;
+    case EW_JAC_UPD: b += 2 * E + Dv; break;
+    case EW_DOTS: b += E; break;
+    case EW_MX_XUPD: b += (double)a.n * nv * 8 + E; break; // two FP32 inputs, z read
+    default: break;
+    }
+    if (nd > 0 && op != EW_RB_R && a.w != nullptr)
+        b += E;
+    return b;
 }
 
 template <typename T>
 void ew(int op, int nd, int nv, const EwArgs<T> &a, const RedArgs &r, const FinArgs<T> &f) {
+    KernelTimer &kt = KernelTimer::get();
+    int tok = -1;
+    if (kt.on()) {
+        char cls[96];
+        snprintf(cls, sizeof(cls), "ew %s op%d nd%d nv%d n%lld", sizeof(T) == 8 ? "f64" : "f32", op, nd,
+                 nv, (long long)a.n);
+        tok = kt.begin(cls, ew_launch_bytes<T>(op, nd, nv, a));
+    }
     launch_ew<T>(op, nd, nv, a, r, f, Runtime::get().stream);
+    kt.end(tok);
 }
 
 template <typename T>

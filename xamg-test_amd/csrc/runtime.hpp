@@ -103,7 +103,7 @@ struct DevMatrix {
     // epilogue, fused dots), chosen by timing the candidates on this matrix with the
     // epilogue's own extra streams (autotune.hip).
     struct Tune {
-        int vpt = 0, grid = 0, remap = -1;
+        int vpt = 0, grid = 0, remap = -1, mode = 0;
         bool valid = false;
     };
     std::map<int, Tune> tune;
@@ -113,6 +113,67 @@ struct DevMatrix {
 // Autotuner switches: XAMG_AUTOTUNE=0 disables it; XAMG_TUNE_FILE names a text file
 // that caches results across processes.
 bool autotune_enabled();
+
+// A 512 MiB device-to-device copy that evicts the L2 caches and the last-level cache.
+struct CacheFlush {
+    static constexpr size_t kBytes = size_t(512) << 20;
+    DBuf src, dst;
+    CacheFlush();
+    void run();
+};
+
+// Mean device time (ms) of one SpMV launch with the launch configuration in `a`: reps
+// back-to-back launches, or with `flush` each launch alone after a cache flush.
+template <typename T>
+float time_spmv_cfg(DevMatrix &A, Prec vp, int nv, int epi, int nd, SpmvArgs<T> a, int reps,
+                    CacheFlush *flush);
+
+// Mean device time (ms) of one launch of SpMV epilogue `epi` with `nd` fused dots on
+// buffers shaped like its operands; cfg = {vpt, grid, remap, mode} or NULL / cfg[0] = 0
+// for the tuned configuration; cold: each launch after a cache flush.
+float bench_spmv_epi(DevMatrix &A, Prec vp, int nv, int epi, int nd, int cold, int iters,
+                     const int *cfg);
+
+// Device time per launch class: HIP events recorded on the launch stream around every
+// SpMV and element-wise dispatch while switched on (a dispatch with fused dots includes
+// its finish kernel), each with the algorithmic HBM bytes of that launch.  Nothing is
+// recorded while the stream is being captured into a graph.
+class KernelTimer {
+  public:
+    struct Stat {
+        std::string name;
+        int64_t calls = 0;
+        double ms = 0, bytes = 0;
+    };
+    static KernelTimer &get();
+    bool on() const { return on_; }
+    // stops recording for its lifetime (launches of the autotuner and probes)
+    struct Pause {
+        bool was;
+        Pause() : was(get().on_) { get().on_ = false; }
+        ~Pause() { get().on_ = was; }
+    };
+    void start();
+    void stop() { on_ = false; }
+    // token for end(), -1 when not recording
+    int begin(const std::string &cls, double bytes);
+    void end(int token);
+    // waits for the stream and folds the recorded launches into the table
+    const std::vector<Stat> &stats();
+
+  private:
+    struct Pending {
+        int stat;
+        double bytes;
+        hipEvent_t e0, e1;
+    };
+    bool on_ = false;
+    std::vector<Stat> st_;
+    std::map<std::string, int> idx_;
+    std::vector<Pending> pend_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t take();
+};
 
 // y = A x dispatch on the value precision of the matrix copy in use.
 template <typename T>

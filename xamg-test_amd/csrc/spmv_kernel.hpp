@@ -1,0 +1,438 @@
+// Device code of the CSR SpMV family (kernel template and its launchers), shared by
+// the translation units that instantiate its staging modes.
+#pragma once
+
+#include "kernels.hpp"
+
+#include <type_traits>
+
+namespace xamg {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+template <typename T, typename V>
+__device__ __forceinline__ T to_t(V v) {
+    return (T)v;
+}
+template <>
+__device__ __forceinline__ float to_t<float, __hip_bfloat16>(__hip_bfloat16 v) {
+    return __bfloat162float(v);
+}
+template <>
+__device__ __forceinline__ double to_t<double, __hip_bfloat16>(__hip_bfloat16 v) {
+    return (double)__bfloat162float(v);
+}
+
+// XCD-aware workgroup -> contiguous-range map (see kernels.hpp).
+__device__ __forceinline__ int logical_block(int b, int G, int remap) {
+    if (remap && (G % kNumXcd) == 0)
+        return (b % kNumXcd) * (G / kNumXcd) + b / kNumXcd;
+    return b;
+}
+
+
+template <typename T, int NV, int ND, int VPT>
+__device__ void reduce_and_finish(double (&acc)[ND][VPT], const RedArgs &r, const FinArgs<T> &f) {
+    constexpr int LR = NV / VPT;
+    __shared__ double lds[kBlock / 64][ND][NV];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int G = gridDim.x;
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+#pragma unroll
+        for (int v = 0; v < VPT; v++) {
+            double s = acc[k][v];
+#pragma unroll
+            for (int off = 32; off >= LR; off >>= 1)
+                s += __shfl_xor(s, off);
+            if (lane < LR)
+                lds[wave][k][lane * VPT + v] = s;
+        }
+    }
+    __syncthreads();
+    if (tid < ND * NV) {
+        const int k = tid / NV, l = tid % NV;
+        double s = lds[0][k][l];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++)
+            s += lds[w][k][l];
+        r.ws.partials[(int64_t)tid * r.ws.max_blocks + blockIdx.x] = s;
+    }
+    (void)G;
+    (void)f;
+}
+
+// Loads / stores of VPT consecutive vector elements (16-byte accesses where possible).
+template <typename T, int VPT>
+__device__ __forceinline__ void ld_vec(const T *__restrict__ p, T (&v)[VPT]) {
+    if constexpr (sizeof(T) * VPT % 16 == 0) {
+        using V4 = uint4;
+#pragma unroll
+        for (int q = 0; q < (int)(sizeof(T) * VPT / 16); q++) {
+            V4 t = ((const V4 *)p)[q];
+            __builtin_memcpy(&v[q * 16 / sizeof(T)], &t, 16);
+        }
+    } else if constexpr (sizeof(T) * VPT == 8) {
+        uint2 t = *(const uint2 *)p;
+        __builtin_memcpy(&v[0], &t, 8);
+    } else {
+#pragma unroll
+        for (int q = 0; q < VPT; q++)
+            v[q] = p[q];
+    }
+}
+
+template <typename T, int VPT>
+__device__ __forceinline__ void st_vec(T *__restrict__ p, const T (&v)[VPT]) {
+    if constexpr (sizeof(T) * VPT % 16 == 0) {
+#pragma unroll
+        for (int q = 0; q < (int)(sizeof(T) * VPT / 16); q++) {
+            uint4 t;
+            __builtin_memcpy(&t, &v[q * 16 / sizeof(T)], 16);
+            ((uint4 *)p)[q] = t;
+        }
+    } else if constexpr (sizeof(T) * VPT == 8) {
+        uint2 t;
+        __builtin_memcpy(&t, &v[0], 8);
+        *(uint2 *)p = t;
+    } else {
+#pragma unroll
+        for (int q = 0; q < VPT; q++)
+            p[q] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV with fused epilogue and optional fused dot products.
+// LR = NV/VPT lanes per row; each lane keeps VPT right-hand sides in registers and
+// reads them with 16-byte loads.  The nonzeros of a row are walked in chunks of U:
+// all column indices, values and x segments of a chunk are loaded first, then added
+// in ascending column order (the reference's order, csr_matvec.inl:213-231).
+//
+// STAGE = 1: the workgroup first copies the column indices and values of its current
+// tile of RPB rows into LDS with consecutive lanes reading consecutive entries (whole
+// cache lines per wavefront load), and the rows' lanes read them from LDS.  Per row the
+// lanes of one row would otherwise read 4-byte entries a row length apart, so one
+// wavefront load touches one cache line per few lanes.  A tile whose nonzeros exceed
+// the LDS capacity is read directly, as with STAGE = 0.
+// ---------------------------------------------------------------------------
+constexpr int kStageCap = 2048; // staged nonzeros per tile (16 per row at 128 rows)
+
+template <typename V, typename T, int NV, int VPT, int EPI, int ND, int STAGE>
+__global__ void __launch_bounds__(kBlock)
+    k_spmv(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
+    if (r.halt != nullptr && *r.halt)
+        return;
+    if (r.gate != nullptr && !*r.gate)
+        return;
+    constexpr int LR = NV / VPT;
+    constexpr int RPB = kBlock / LR;
+    constexpr int U = (16 / VPT) > 2 ? (16 / VPT) : 2;
+    const V *__restrict__ val = (const V *)a.val;
+    const uint32_t *__restrict__ row = a.row;
+    const uint32_t *__restrict__ col = a.col;
+    const T *__restrict__ x = a.x;
+    const int tid = threadIdx.x, lr = tid / LR, sub = tid % LR;
+    const int l0 = sub * VPT;
+    const int G = gridDim.x;
+    const int L = logical_block(blockIdx.x, G, remap);
+    const int64_t r0 = a.n * L / G, r1 = a.n * (L + 1) / G;
+    constexpr int NDX = ND > 0 ? ND : 1;
+    double dacc[NDX][VPT];
+#pragma unroll
+    for (int k = 0; k < NDX; k++)
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            dacc[k][v] = 0.0;
+    constexpr int CAP = STAGE ? kStageCap : 1;
+    __shared__ uint32_t s_col[CAP];
+    __shared__ V s_val[CAP];
+    for (int64_t base = r0; base < r1; base += RPB) {
+        const int64_t i = base + lr;
+        uint32_t jb = 0;
+        bool staged = false;
+        if constexpr (STAGE) {
+            const int64_t bend = base + RPB < r1 ? base + RPB : r1;
+            const uint32_t jA = row[base], jB = row[bend];
+            staged = jB - jA <= (uint32_t)CAP;
+            jb = jA;
+            __syncthreads(); // the previous tile's rows are done with the LDS copy
+            if (staged)
+                for (uint32_t k = tid; k < jB - jA; k += kBlock) {
+                    s_col[k] = col[jA + k];
+                    s_val[k] = val[jA + k];
+                }
+            __syncthreads();
+        }
+        if (i < r1) {
+            const int64_t e = i * NV + l0;
+            const uint32_t j0 = row[i], j1 = row[i + 1];
+            T acc[VPT];
+            if constexpr (EPI == EPI_AXPY) {
+                ld_vec<T, VPT>(a.y + e, acc);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    acc[v] = (T)0;
+            }
+            // the epilogue's own row operands are loaded before the gathers so their
+            // latency overlaps the nonzero loop
+            constexpr bool kNeedB = EPI != EPI_AXY && EPI != EPI_AXPY;
+            constexpr bool kNeedD = EPI == EPI_JACOBI || EPI == EPI_CHEB || EPI == EPI_CHEB_LAST;
+            T pb[VPT], pw[VPT], px[VPT];
+            T pdi = (T)0;
+            if constexpr (kNeedB)
+                ld_vec<T, VPT>(a.b + e, pb);
+            if constexpr (kNeedD)
+                pdi = a.d[i];
+            if constexpr (EPI == EPI_JACOBI) {
+                ld_vec<T, VPT>(a.xin + e, px);
+            } else if constexpr (EPI == EPI_CHEB_LAST) {
+                if (a.xin != nullptr)
+                    ld_vec<T, VPT>(a.xin + e, px);
+            }
+            if constexpr (ND == 3) {
+                ld_vec<T, VPT>(a.w + e, pw);
+            } else if constexpr (ND == 1) {
+                if (a.w != nullptr)
+                    ld_vec<T, VPT>(a.w + e, pw);
+            }
+            for (uint32_t j = j0; j < j1; j += U) {
+                uint32_t c[U];
+                T av[U];
+                T xv[U][VPT];
+                if (STAGE && staged) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const bool ok = j + u < j1;
+                        const uint32_t q = (ok ? j + u : j0) - jb;
+                        c[u] = s_col[q];
+                        av[u] = ok ? to_t<T>(s_val[q]) : (T)0;
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const bool ok = j + u < j1;
+                        c[u] = ok ? col[j + u] : col[j0];
+                        av[u] = ok ? to_t<T>(val[j + u]) : (T)0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    ld_vec<T, VPT>(x + (int64_t)c[u] * NV + l0, xv[u]);
+                if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
+                    if (a.x_is_rhs && a.prescaled) {
+                        // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const T dc = a.dcol[c[u]];
+#pragma unroll
+                            for (int v = 0; v < VPT; v++)
+                                xv[u][v] = a.xcoef * (xv[u][v] * dc);
+                        }
+                    } else if (a.x_is_rhs) {
+                        // first Horner step read straight from the right-hand side:
+                        // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const T dc = a.dcol[c[u]];
+#pragma unroll
+                            for (int v = 0; v < VPT; v++)
+                                xv[u][v] = (a.xcoef * (xv[u][v] * dc)) * dc;
+                        }
+                    } else if (!a.prescaled) {
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const T dc = a.dcol[c[u]];
+#pragma unroll
+                            for (int v = 0; v < VPT; v++)
+                                xv[u][v] = xv[u][v] * dc;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j + u < j1)
+#pragma unroll
+                        for (int v = 0; v < VPT; v++)
+                            acc[v] = acc[v] + xv[u][v] * av[u];
+            }
+            T out[VPT];
+            if constexpr (EPI == EPI_AXY || EPI == EPI_AXPY) {
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    out[v] = acc[v];
+            } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    out[v] = (T)1.0 * pb[v] + (T)(-1.0) * acc[v];
+            } else if constexpr (EPI == EPI_JACOBI) {
+                const T di = pdi;
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    out[v] = px[v] + (a.wcoef[l0 + v] * di) * (pb[v] - acc[v]);
+            } else {
+                const T di = pdi;
+#pragma unroll
+                for (int v = 0; v < VPT; v++) {
+                    const T rr = pb[v] * di;
+                    const T t2 = a.prescaled ? acc[v] : acc[v] * di;
+                    out[v] = a.coef * rr + (T)1.0 * t2;
+                }
+                if constexpr (EPI == EPI_CHEB_LAST) {
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        out[v] = out[v] * di;
+                    if (a.xin != nullptr) {
+#pragma unroll
+                        for (int v = 0; v < VPT; v++)
+                            out[v] = (T)1.0 * out[v] + (T)1.0 * px[v];
+                    }
+                }
+            }
+            if constexpr (EPI == EPI_RESID_MX) {
+                float o32[VPT];
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    o32[v] = (float)out[v];
+                st_vec<float, VPT>(a.yo + e, o32);
+                if (a.yo2 != nullptr)
+                    st_vec<float, VPT>(a.yo2 + e, o32);
+                if (a.yo3 != nullptr)
+                    st_vec<float, VPT>(a.yo3 + e, o32);
+            } else {
+                st_vec<T, VPT>(a.y + e, out);
+                if (a.y2 != nullptr)
+                    st_vec<T, VPT>(a.y2 + e, out);
+                if (a.y3 != nullptr)
+                    st_vec<T, VPT>(a.y3 + e, out);
+            }
+            if constexpr (ND == 1) {
+                if (a.w != nullptr) {
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        dacc[0][v] += (double)out[v] * (double)pw[v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        dacc[0][v] += (double)out[v] * (double)out[v];
+                }
+            } else if constexpr (ND == 3) {
+#pragma unroll
+                for (int v = 0; v < VPT; v++) {
+                    const double o = (double)out[v], wv = (double)pw[v];
+                    dacc[0][v] += o * wv;
+                    dacc[1][v] += o * o;
+                    dacc[2][v] += wv * wv;
+                }
+            }
+        }
+    }
+    if constexpr (ND > 0)
+        reduce_and_finish<T, NV, ND, VPT>(dacc, r, f);
+}
+
+template <typename V, typename T, int NV, int VPT, int STAGE>
+void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
+                     hipStream_t s) {
+    LaunchCfg c = launch_cfg();
+    if (a.cfg_grid > 0) {
+        c.grid_spmv = a.cfg_grid;
+        c.grid_red = a.cfg_grid;
+    }
+    if (a.cfg_remap >= 0)
+        c.xcd_remap = a.cfg_remap;
+    int G = nd > 0 ? c.grid_red : c.grid_spmv;
+    if (nd > 0 && G > r.ws.max_blocks)
+        G = r.ws.max_blocks;
+    constexpr int LR = NV / VPT;
+    int64_t need = (a.n * LR + kBlock - 1) / kBlock;
+    if (need < G)
+        G = (int)(need > 0 ? need : 1);
+    dim3 g(G), b(kBlock);
+#define XAMG_SPMV_CASE(E, D)                                                                       \
+    if (epi == E && nd == D) {                                                                     \
+        hipLaunchKernelGGL((k_spmv<V, T, NV, VPT, E, D, STAGE>), g, b, 0, s, a, r, f, c.xcd_remap);       \
+        launch_finish<T, NV>(D, r, f, G, s);                                                       \
+        return;                                                                                    \
+    }
+    XAMG_SPMV_CASE(EPI_AXY, 0)
+    XAMG_SPMV_CASE(EPI_AXY, 1)
+    XAMG_SPMV_CASE(EPI_AXY, 3)
+    XAMG_SPMV_CASE(EPI_AXPY, 0)
+    XAMG_SPMV_CASE(EPI_RESID, 0)
+    XAMG_SPMV_CASE(EPI_RESID, 1)
+    XAMG_SPMV_CASE(EPI_JACOBI, 0)
+    XAMG_SPMV_CASE(EPI_CHEB, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_LAST, 0)
+    if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value) {
+        XAMG_SPMV_CASE(EPI_RESID_MX, 0)
+        XAMG_SPMV_CASE(EPI_RESID_MX, 1)
+    }
+#undef XAMG_SPMV_CASE
+    throw std::runtime_error("xamg: unsupported spmv epilogue/dot combination");
+}
+
+template <typename V, typename T, int NV, int STAGE>
+void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinArgs<T> &f,
+                    hipStream_t s) {
+    int vpt = a.cfg_vpt > 0 ? a.cfg_vpt : spmv_vpt_choice(NV, (int)sizeof(T));
+    if constexpr (NV == 1) {
+        spmv_nvv<V, T, 1, 1, STAGE>(epi, nd, a, r, f, s);
+    } else if constexpr (NV == 2) {
+        if (vpt == 1)
+            spmv_nvv<V, T, 2, 1, STAGE>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 2, 2, STAGE>(epi, nd, a, r, f, s);
+    } else if constexpr (NV == 4) {
+        if (vpt == 2)
+            spmv_nvv<V, T, 4, 2, STAGE>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 4, 4, STAGE>(epi, nd, a, r, f, s);
+    } else if constexpr (NV == 8) {
+        if (vpt == 2)
+            spmv_nvv<V, T, 8, 2, STAGE>(epi, nd, a, r, f, s);
+        else if (vpt == 8)
+            spmv_nvv<V, T, 8, 8, STAGE>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 8, 4, STAGE>(epi, nd, a, r, f, s);
+    } else {
+        if (vpt == 8)
+            spmv_nvv<V, T, 16, 8, STAGE>(epi, nd, a, r, f, s);
+        else if (vpt == 16)
+            spmv_nvv<V, T, 16, 16, STAGE>(epi, nd, a, r, f, s);
+        else
+            spmv_nvv<V, T, 16, 4, STAGE>(epi, nd, a, r, f, s);
+    }
+}
+
+// Launch of the SpMV family for one staging mode (0: column indices and values read by
+// each row's lanes, 1: staged through LDS per tile of rows).
+template <typename V, typename T, int STAGE>
+void launch_spmv_mode(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                      const FinArgs<T> &f, hipStream_t s) {
+    switch (nv) {
+    case 1:
+        spmv_nv<V, T, 1, STAGE>(epi, ndots, a, r, f, s);
+        break;
+    case 2:
+        spmv_nv<V, T, 2, STAGE>(epi, ndots, a, r, f, s);
+        break;
+    case 4:
+        spmv_nv<V, T, 4, STAGE>(epi, ndots, a, r, f, s);
+        break;
+    case 8:
+        spmv_nv<V, T, 8, STAGE>(epi, ndots, a, r, f, s);
+        break;
+    case 16:
+        spmv_nv<V, T, 16, STAGE>(epi, ndots, a, r, f, s);
+        break;
+    default:
+        throw std::runtime_error("xamg: nv must be one of 1, 2, 4, 8, 16");
+    }
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace xamg

@@ -104,11 +104,16 @@ def lib():
             "xamgx_dot": (ci, [sz, sz, ci, vp]),
             "xamgx_bench_spmv": (ci, [sz, ci, ci, ci, vp]),
             "xamgx_bench_copy": (ci, [C.c_uint64, ci, vp]),
+            "xamgx_bench_spmv_epi": (ci, [sz, ci, ci, ci, ci, ci, ci, vp, vp]),
+            "xamgx_kernel_timing": (ci, [ci]),
+            "xamgx_kernel_stats_count": (ci, [vp]),
+            "xamgx_kernel_stats_get": (ci, [ci, cp, ci, vp, vp, vp]),
             "xamgx_solver_last_ms": (ci, [sz, vp]),
             "xamgx_solver_get_history": (ci, [sz, vp, ci, vp]),
             "xamgx_solver_get_residuals": (ci, [sz, vp, vp]),
             "xamgx_set_engine": (ci, [ci, ci, ci, ci, ci, ci]),
             "xamgx_set_spmv_vpt": (ci, [ci, ci, ci]),
+            "xamgx_set_spmv_stage": (ci, [ci]),
             "xamgx_matrix_set_eigs": (ci, [sz, cd, cd]),
             "xamgx_matrix_get_eigs": (ci, [sz, vp, vp]),
             "xamgx_hierarchy_build": (ci, [sz, sz, psz]),
@@ -284,6 +289,15 @@ class Matrix:
     def bench_spmv(self, nv, prec=0, iters=100):
         ms = C.c_double()
         check(lib().xamgx_bench_spmv(self.id, prec, nv, iters, C.byref(ms)))
+        return ms.value
+
+    def bench_spmv_epi(self, nv, prec=0, epi=0, nd=0, cold=True, iters=5, cfg=None):
+        """Device ms of one launch of an SpMV epilogue (see xamgx_bench_spmv_epi);
+        cfg = (vpt, grid, remap, mode) or None for the autotuned choice."""
+        ms = C.c_double()
+        arr = None if cfg is None else (C.c_int * 4)(*cfg)
+        check(lib().xamgx_bench_spmv_epi(self.id, prec, nv, epi, nd, int(bool(cold)), iters,
+                                         arr, C.byref(ms)))
         return ms.value
 
     def destroy(self):
@@ -564,10 +578,38 @@ def set_spmv_vpt(nv, es, vpt):
     check(lib().xamgx_set_spmv_vpt(nv, es, vpt))
 
 
+def set_spmv_stage(mode):
+    """LDS staging in every SpMV: -1 as autotuned, 0 off, 1 on."""
+    check(lib().xamgx_set_spmv_stage(mode))
+
+
 def bench_copy(nbytes, iters=20):
     g = C.c_double()
     check(lib().xamgx_bench_copy(nbytes, iters, C.byref(g)))
     return g.value
+
+
+def kernel_timing(on):
+    """Start (clearing the record) or stop per-launch device timing of the solve path."""
+    check(lib().xamgx_kernel_timing(int(bool(on))))
+
+
+def kernel_stats():
+    """Recorded launch classes: name, calls, total ms, total algorithmic bytes, average
+    microseconds per launch and achieved GB/s, ordered by total time."""
+    cnt = C.c_int()
+    check(lib().xamgx_kernel_stats_count(C.byref(cnt)))
+    out = []
+    buf = C.create_string_buffer(256)
+    for i in range(cnt.value):
+        calls, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        check(lib().xamgx_kernel_stats_get(i, buf, 256, C.byref(calls), C.byref(ms), C.byref(by)))
+        k = calls.value
+        out.append({"name": buf.value.decode(), "calls": k, "ms": ms.value, "bytes": by.value,
+                    "avg_us": 1e3 * ms.value / k if k else 0.0,
+                    "gbs": by.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0})
+    out.sort(key=lambda r: -r["ms"])
+    return out
 
 
 def device_info():
