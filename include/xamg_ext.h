@@ -92,8 +92,10 @@ int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int 
 /* Right-hand sides per lane of the SpMV kernel for nv and element size es (8: FP64
  * vectors, 4: FP32 vectors); 0 restores the default.  Used by the autotuner. */
 int xamgx_set_spmv_vpt(int nv, int es, int vpt);
-/* LDS staging of each row tile's column indices and values in every SpMV launch:
- * -1 as autotuned (default), 0 off, 1 on.  Results are identical either way. */
+/* Staging mode of every SpMV launch: -1 as autotuned (default), 0 direct loads, 1 each
+ * row tile's column indices and values staged through LDS, 2 as 1 with the streamed
+ * operands loaded non-temporally, 3 double-buffered staging by direct global-to-LDS
+ * loads (BF16 values: as 1).  Results are identical in every mode. */
 int xamgx_set_spmv_stage(int mode);
 /* Spectrum bounds of D^{-1/2} A D^{-1/2} used by Chebyshev (the reference takes
  * them from hypre's setup, hypre_base.inl:369-375). */

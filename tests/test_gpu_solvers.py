@@ -25,7 +25,11 @@ def _params(method, prec=None, tol=1e-10, max_iters=500, extra=None):
     return p
 
 
-PERTURB_CHUNKS = (2, 3, 7, 16)
+PERTURB_CHUNKS = (2, 3, 7, 16, 64)
+# Past the check where the oracle's own histories under these summation orders differ by
+# this fraction, residual histories are not reproducible (the reference on another
+# number of cores gives another one); only iterations and convergence are compared there.
+CHAOS = 0.5
 
 
 def _hist_dev(a, b, nv):
@@ -43,9 +47,10 @@ def _hist_dev(a, b, nv):
 def _compare(gpu, orc, nv, perturbed=None):
     """Iterations within 2, same convergence flags, and a residual history that stays as
     close to the oracle's as the oracle itself stays under a change of dot-product
-    summation order: `perturbed` holds the oracle rerun with 2/3/7/16-chunk dots (the
+    summation order: `perturbed` holds the oracle rerun with 2/3/7/16/64-chunk dots (the
     reference on ranks of that many cores).  Per check, the GPU deviation must not
-    exceed 10x the running maximum of those deviations (plus 1e-10)."""
+    exceed 10x the running maximum of those deviations (plus 1e-10), up to the check
+    where that maximum reaches CHAOS."""
     assert abs(gpu["iters"] - orc["iters"]) <= 2, (gpu["iters"], orc["iters"])
     assert gpu["converged"] == orc["converged"]
     hg, ho = gpu["hist"], orc["hist"]
@@ -63,7 +68,7 @@ def _compare(gpu, orc, nv, perturbed=None):
     dp = np.fmax.accumulate(dp, axis=0)
     for it in range(k):
         for l in range(nv):
-            if np.isnan(dg[it, l]):
+            if np.isnan(dg[it, l]) or dp[it, l] >= CHAOS:
                 continue
             assert dg[it, l] <= 10 * dp[it, l] + 1e-10, (it, l, dg[it, l], dp[it, l])
 

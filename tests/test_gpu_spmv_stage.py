@@ -1,7 +1,8 @@
-"""The LDS-staged SpMV (column indices and values of each row tile copied to LDS first)
-against the oracle, bit for bit, at every compiled RHS-per-lane width; tiles whose
-nonzeros exceed the LDS capacity take the direct path inside the same launch; a solve
-gives identical iterates with staging forced on and off."""
+"""The LDS-staged SpMV modes (1: column indices and values of each row tile copied to
+LDS first; 2: the same with non-temporal loads of the streamed operands; 3: double-
+buffered staging by direct global-to-LDS loads) against the oracle, bit for bit, at
+every compiled RHS-per-lane width; tiles whose nonzeros exceed the LDS capacity take the
+direct path inside the same launch; a solve gives identical iterates in every mode."""
 import numpy as np
 import pytest
 
@@ -13,9 +14,9 @@ pytestmark = pytest.mark.gpu
 VPTS = {1: [1], 2: [1, 2], 4: [2, 4], 8: [2, 4, 8], 16: [4, 8, 16]}
 
 
-@pytest.fixture
-def staged(xamg):
-    xamg.set_spmv_stage(1)
+@pytest.fixture(params=[1, 2, 3])
+def staged(xamg, request):
+    xamg.set_spmv_stage(request.param)
     yield xamg
     xamg.set_spmv_stage(-1)
     for nv, vs in VPTS.items():
@@ -36,7 +37,7 @@ def test_staged_fp64_fp32_bitexact(staged, nv):
     y0 = X.rhs(n, nv, 4)
     refax = O.spmv(O.csr(row, col, val), x, nv, axpy_into=y0.copy())
     for vpt in VPTS[nv]:
-        print(f"stage=1 nv={nv} vpt={vpt}", flush=True)
+        print(f"nv={nv} vpt={vpt}", flush=True)
         X.set_spmv_vpt(nv, 8, vpt)
         X.set_spmv_vpt(nv, 4, vpt)
         y = A.spmv(x, nv, prec=0)
@@ -101,12 +102,13 @@ def test_staged_solve_identical_iterates(xamg):
                                  "spectrum_fraction": 0.3}}
     outs = []
     try:
-        for mode in (0, 1):
+        for mode in (0, 1, 2, 3):
             xamg.set_spmv_stage(mode)
             outs.append(xamg.solve_system(row, col, val, b, nv, params, eigs=(1 - c, 1 + c)))
     finally:
         xamg.set_spmv_stage(-1)
-    a, s = outs
-    assert a["iters"] == s["iters"]
-    assert np.array_equal(a["x"].view(np.uint64), s["x"].view(np.uint64))
-    assert np.array_equal(a["hist"], s["hist"])
+    a = outs[0]
+    for s in outs[1:]:
+        assert a["iters"] == s["iters"]
+        assert np.array_equal(a["x"].view(np.uint64), s["x"].view(np.uint64))
+        assert np.array_equal(a["hist"], s["hist"])

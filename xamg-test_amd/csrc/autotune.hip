@@ -1,7 +1,9 @@
 // SpMV autotuner: for each (matrix, value precision, vector precision, nv, epilogue,
 // fused dots) it times the compiled RHS-per-lane widths (which fix rows per wavefront =
 // 64 * VPT / nv and the nonzero unroll 16 / VPT), the workgroup counts, the XCD-aware row
-// mapping and the LDS staging of column indices and values on the matrix itself, and
+// mapping and the staging modes (LDS staging of column indices and values, non-temporal
+// loads of the streamed operands, double-buffered staging by direct global-to-LDS
+// loads) on the matrix itself, and
 // keeps the fastest.  Each candidate launch is timed cold: a 512 MiB copy before it
 // evicts the L2 caches and the 256 MB last-level cache, as the other vectors of a
 // solver iteration do between two SpMVs (timed back to back, the matrix and vectors of a
@@ -157,15 +159,22 @@ DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
     SpmvArgs<T> a = bufs.args(epi);
     DevMatrix::Tune best;
     float best_ms = 1e30f;
-    for (int mode : {0, 1})
-        for (int vpt : vpt_options(nv))
-            for (int grid : {1024, 2048, 4096, 8192})
+    // With fused dots the RHS-per-lane width and the workgroup count fix the order in
+    // which the dot products are summed, so they stay at the defaults there and the
+    // results do not depend on the tuning; the XCD mapping and the staging mode do not
+    // change the order.
+    const std::vector<int> vpts = nd > 0 ? std::vector<int>{0} : vpt_options(nv);
+    const std::vector<int> grids =
+        nd > 0 ? std::vector<int>{launch_cfg().grid_red} : std::vector<int>{1024, 2048, 4096, 8192};
+    for (int mode : {0, 1, 2, 3})
+        for (int vpt : vpts)
+            for (int grid : grids)
                 for (int remap : {0, 1}) {
                     a.cfg_vpt = vpt;
                     a.cfg_grid = grid;
                     a.cfg_remap = remap;
                     a.cfg_mode = mode;
-                    const float ms = time_spmv_cfg<T>(A, vp, nv, epi, nd, a, 3, &flush);
+                    const float ms = time_spmv_cfg<T>(A, vp, nv, epi, nd, a, 4, &flush);
                     if (ms < best_ms) {
                         best_ms = ms;
                         best.vpt = vpt;
@@ -190,7 +199,7 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
     if (!autotune_enabled() || n < 65536)
         return t;
     std::ostringstream k;
-    k << "cold2_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
+    k << "cold5_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
       << nv << "_epi" << epi << "_nd" << nd;
     const std::string key = k.str();
     auto &c = cache();

@@ -363,7 +363,10 @@ void launch_finish(int nd, const RedArgs &r, const FinArgs<T> &f, int G, hipStre
 }
 
 // ---------------------------------------------------------------------------
-// Element-wise kernels (row ranges, one lane per (row, RHS) element).
+// Element-wise kernels (row ranges, one lane per (row, RHS) element).  Every operand of
+// an element is loaded before the first use of any of them, without branches between
+// the loads, so each element costs one round trip to memory; the rare keep / copy /
+// freeze cases of the two-term update take a separate loop.
 // ---------------------------------------------------------------------------
 template <typename T, int NV, int OP, int ND>
 __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
@@ -377,6 +380,7 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
     }
     if (r.gate != nullptr && !*r.gate)
         return;
+    using O = typename std::conditional<std::is_same<T, double>::value, float, double>::type;
     const bool keep = a.keep_if != nullptr && *a.keep_if;
     const bool copy = a.copy_if != nullptr && *a.copy_if;
     const int tid = threadIdx.x;
@@ -388,85 +392,127 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
     const T ca = a.a != nullptr ? a.a[l] : (T)1.0;
     const T cb = a.b != nullptr ? a.b[l] : (T)1.0;
     const T cc = a.c != nullptr ? a.c[l] : (T)1.0;
+    T ao = (T)0, bo = (T)0;
     bool frozen = false;
-    if constexpr (OP == EW_AXPBYPCZ)
+    if constexpr (OP == EW_AXPBYPCZ) {
         frozen = a.freeze_zero && ca == (T)0 && cb == (T)0;
-    else if constexpr (OP == EW_AXPBY_Z)
+    } else if constexpr (OP == EW_AXPBY_Z) {
         frozen = a.freeze_zero && ca == (T)0;
-    else if constexpr (OP == EW_MX_XUPD)
+    } else if constexpr (OP == EW_MX_XUPD) {
+        ao = (T)a.ao[l];
+        bo = (T)a.bo[l];
         frozen = a.freeze_zero && a.ao[l] == 0.0f && a.bo[l] == 0.0f;
+    }
+    // second dot operand: w, or (ND = 1 without w) the result itself; the fallback
+    // pointer only keeps the load unconditional
+    const bool has_w = a.w != nullptr;
+    const T *wp = has_w ? a.w : a.x;
     double dacc[ND > 0 ? ND : 1][1];
 #pragma unroll
     for (int k = 0; k < (ND > 0 ? ND : 1); k++)
         dacc[k][0] = 0.0;
-    for (int64_t e = e0 + tid; e < e1; e += kBlock) {
-        T out;
-        if constexpr (OP == EW_FILL) {
-            out = a.c0;
-        } else if constexpr (OP == EW_AXPBY_Z || OP == EW_RB_R) {
-            out = (keep || frozen) ? a.z[e] : ca * a.x[e] + cb * a.y[e];
-            if (copy && a.y2 != nullptr)
-                a.y2[e] = out;
-        } else if constexpr (OP == EW_AXPBYPCZ) {
-            if (copy)
-                out = a.x[e];
-            else
-                out = (frozen || keep) ? a.z[e] : ca * a.x[e] + cb * a.y[e] + cc * a.z[e];
-        } else if constexpr (OP == EW_JACOBI0) {
-            const int64_t i = e / NV;
-            out = (ca * a.d[i]) * a.x[e];
-        } else if constexpr (OP == EW_CHEB0 || OP == EW_CHEB0_LAST) {
-            const int64_t i = e / NV;
-            const T di = a.d[i];
-            out = a.c0 * (a.x[e] * di);
-            if constexpr (OP == EW_CHEB0_LAST) {
-                out = out * di;
-                if (a.y != nullptr)
-                    out = (T)1.0 * out + (T)1.0 * a.y[e];
-            }
-        } else if constexpr (OP == EW_COPY3) {
-            out = (T)1.0 * a.x[e];
-            if (a.y2 != nullptr)
-                a.y2[e] = out;
-            if (a.y3 != nullptr)
-                a.y3[e] = out;
-        } else if constexpr (OP == EW_ADD_CONV) {
-            using O = typename std::conditional<std::is_same<T, double>::value, float, double>::type;
-            out = (T)1.0 * (T)((const O *)a.xo)[e] + (T)1.0 * a.z[e];
-        } else if constexpr (OP == EW_CONVERT) {
-            using O = typename std::conditional<std::is_same<T, double>::value, float, double>::type;
-            out = (T)((const O *)a.xo)[e];
-        } else if constexpr (OP == EW_SCAL_D) {
-            out = a.z[e] * a.d[e / NV];
-        } else if constexpr (OP == EW_JAC_UPD) {
-            out = (ca * cc) * (a.y[e] * a.d[e / NV]) + (T)1.0 * a.z[e];
-        } else if constexpr (OP == EW_DOTS) {
-            out = a.x[e];
-        } else { // EW_MX_XUPD: FP64 accumulation of FP32 directions
-            out = frozen ? a.z[e]
-                         : (T)a.ao[l] * (T)((const float *)a.xo)[e] +
-                               (T)a.bo[l] * (T)((const float *)a.yo)[e] + (T)1.0 * a.z[e];
-        }
-        if constexpr (OP != EW_DOTS)
-            a.z[e] = out;
+    auto dots = [&](int64_t e, T out, T we, T w2e) {
         if constexpr (OP == EW_RB_R) {
-            const double o = (double)out, wv = (double)a.w[e], w2 = (double)a.w2[e];
+            const double o = (double)out, wv = (double)we, w2 = (double)w2e;
             dacc[0][0] += wv * o;
             dacc[1][0] += wv * wv;
             dacc[2][0] += wv * w2;
             dacc[3][0] += o * o;
         } else if constexpr (ND == 1) {
-            const double wv = (a.w != nullptr && !copy) ? (double)a.w[e] : (double)out;
+            const double wv = (has_w && !copy) ? (double)we : (double)out;
             dacc[0][0] += (double)out * wv;
         } else if constexpr (ND == 3) {
-            const double o = (double)out, wv = copy ? o : (double)a.w[e];
+            const double o = (double)out, wv = copy ? o : (double)we;
             dacc[0][0] += o * wv;
             dacc[1][0] += o * o;
             dacc[2][0] += wv * wv;
         }
+        (void)e;
+    };
+    if constexpr (OP == EW_AXPBY_Z || OP == EW_RB_R) {
+        // z = a x + b y; z kept while *keep_if or for a frozen RHS; copied to y2 while
+        // *copy_if
+        auto run = [&](auto general) {
+            constexpr bool GEN = decltype(general)::value;
+            for (int64_t e = e0 + tid; e < e1; e += kBlock) {
+                const T xe = a.x[e], ye = a.y[e];
+                T we = (T)0, w2e = (T)0, ze = (T)0;
+                if constexpr (OP == EW_RB_R) {
+                    we = a.w[e];
+                    w2e = a.w2[e];
+                } else if constexpr (ND > 0) {
+                    we = wp[e];
+                }
+                if constexpr (GEN)
+                    ze = a.z[e];
+                T out = ca * xe + cb * ye;
+                if constexpr (GEN) {
+                    if (keep || frozen)
+                        out = ze;
+                    if (copy && a.y2 != nullptr)
+                        a.y2[e] = out;
+                }
+                a.z[e] = out;
+                dots(e, out, we, w2e);
+            }
+        };
+        if (keep || copy || a.freeze_zero)
+            run(std::true_type{});
+        else
+            run(std::false_type{});
+    } else {
+        for (int64_t e = e0 + tid; e < e1; e += kBlock) {
+            T we = (T)0;
+            if constexpr (ND > 0)
+                we = wp[e];
+            T out;
+            if constexpr (OP == EW_FILL) {
+                out = a.c0;
+            } else if constexpr (OP == EW_AXPBYPCZ) {
+                const T xe = a.x[e], ye = a.y[e], ze = a.z[e];
+                const T upd = ca * xe + cb * ye + cc * ze;
+                out = copy ? xe : ((frozen || keep) ? ze : upd);
+            } else if constexpr (OP == EW_JACOBI0) {
+                const int64_t i = e / NV;
+                out = (ca * a.d[i]) * a.x[e];
+            } else if constexpr (OP == EW_CHEB0 || OP == EW_CHEB0_LAST) {
+                const int64_t i = e / NV;
+                const T di = a.d[i];
+                out = a.c0 * (a.x[e] * di);
+                if constexpr (OP == EW_CHEB0_LAST) {
+                    out = out * di;
+                    if (a.y != nullptr)
+                        out = (T)1.0 * out + (T)1.0 * a.y[e];
+                }
+            } else if constexpr (OP == EW_COPY3) {
+                out = (T)1.0 * a.x[e];
+                if (a.y2 != nullptr)
+                    a.y2[e] = out;
+                if (a.y3 != nullptr)
+                    a.y3[e] = out;
+            } else if constexpr (OP == EW_ADD_CONV) {
+                out = (T)1.0 * (T)((const O *)a.xo)[e] + (T)1.0 * a.z[e];
+            } else if constexpr (OP == EW_CONVERT) {
+                out = (T)((const O *)a.xo)[e];
+            } else if constexpr (OP == EW_SCAL_D) {
+                out = a.z[e] * a.d[e / NV];
+            } else if constexpr (OP == EW_JAC_UPD) {
+                out = (ca * cc) * (a.y[e] * a.d[e / NV]) + (T)1.0 * a.z[e];
+            } else if constexpr (OP == EW_DOTS) {
+                out = a.x[e];
+            } else { // EW_MX_XUPD: FP64 accumulation of FP32 directions
+                const T ze = a.z[e];
+                const T xe = (T)((const float *)a.xo)[e], ye = (T)((const float *)a.yo)[e];
+                const T upd = ao * xe + bo * ye + (T)1.0 * ze;
+                out = frozen ? ze : upd;
+            }
+            if constexpr (OP != EW_DOTS)
+                a.z[e] = out;
+            dots(e, out, we, (T)0);
+        }
     }
     if constexpr (ND > 0)
-        reduce_and_finish<T, NV, ND, 1>(dacc, r, f);
+        reduce_and_finish<T, NV, ND, 1>(dacc, r, f, L);
 }
 
 // Dense GEMV for the coarse Direct solve (dense_matvec.inl:100-117): one lane per
@@ -565,7 +611,9 @@ int &spmv_vpt_choice(int nv, int es) {
 template <typename V, typename T>
 void launch_spmv(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                  const FinArgs<T> &f, hipStream_t s) {
-    if (a.cfg_mode == 1)
+    if (a.cfg_mode == 3)
+        launch_spmv_stage3<V, T>(epi, ndots, nv, a, r, f, s);
+    else if (a.cfg_mode == 1 || a.cfg_mode == 2)
         launch_spmv_staged<V, T>(epi, ndots, nv, a, r, f, s);
     else
         launch_spmv_mode<V, T, 0>(epi, ndots, nv, a, r, f, s);

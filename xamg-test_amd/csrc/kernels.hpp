@@ -120,7 +120,9 @@ struct SpmvArgs {
     // EPI_RESID_MX: FP32 copies of the FP64 residual (y is not written)
     float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr;
     // launch configuration chosen by the autotuner (0 / -1: global defaults); cfg_mode 1
-    // stages each row tile's column indices and values through LDS
+    // stages each row tile's column indices and values through LDS, 2 also loads the
+    // streamed operands non-temporally, 3 double-buffers the staging with direct
+    // global-to-LDS loads
     int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1, cfg_mode = 0;
 };
 
@@ -154,7 +156,7 @@ struct LaunchCfg {
     int grid_ew = 2048;
     int grid_red = 2048;
     int xcd_remap = 1;
-    int spmv_stage = -1; // SpMV LDS staging: -1 as tuned, 0 off, 1 on for every launch
+    int spmv_stage = -1; // SpMV staging mode for every launch: -1 as tuned, else 0..3
 };
 
 LaunchCfg &launch_cfg();
@@ -173,9 +175,17 @@ template <typename T>
 void launch_ew(int op, int ndots, int nv, const EwArgs<T> &a, const RedArgs &r,
                const FinArgs<T> &f, hipStream_t s);
 
-// The LDS-staged SpMV instantiations (spmv_staged.hip).
+// The LDS-staged SpMV instantiations (spmv_staged.hip: cfg_mode 1; spmv_stage2.hip:
+// cfg_mode 2, streamed operands loaded non-temporally; spmv_stage3.hip: cfg_mode 3,
+// double-buffered staging by direct global-to-LDS loads).
 template <typename V, typename T>
 void launch_spmv_staged(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                        const FinArgs<T> &f, hipStream_t s);
+template <typename V, typename T>
+void launch_spmv_stage2(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                        const FinArgs<T> &f, hipStream_t s);
+template <typename V, typename T>
+void launch_spmv_stage3(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                         const FinArgs<T> &f, hipStream_t s);
 
 // Final pass of a fused reduction over G workgroup partials (nd dot products per RHS),

@@ -342,9 +342,7 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
     a.col = A.col.as<uint32_t>();
     a.n = A.n;
     if (a.cfg_vpt == 0 && a.cfg_grid == 0) {
-        // the scaled FP32 values share the FP32 launch configuration (same bytes)
-        const DevMatrix::Tune &t =
-            A.tuned(vp == Prec::F32S ? Prec::F32 : vp, (int)sizeof(T), nv, epi, nd);
+        const DevMatrix::Tune &t = A.tuned(vp, (int)sizeof(T), nv, epi, nd);
         a.cfg_vpt = t.vpt;
         a.cfg_grid = t.grid;
         a.cfg_remap = t.remap;
@@ -358,7 +356,7 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
         char cls[160];
         // one class per compiled kernel and matrix (k_spmv<V, T, nv, vpt, epilogue, dots>)
         snprintf(cls, sizeof(cls), "spmv %s/%s %s nd%d nv%d stage%d vpt%d n%lld", prec_tag(vp),
-                 sizeof(T) == 8 ? "f64" : "f32", epi_tag(epi), nd, nv, a.cfg_mode == 1 ? 1 : 0,
+                 sizeof(T) == 8 ? "f64" : "f32", epi_tag(epi), nd, nv, a.cfg_mode,
                  eff_vpt(nv, a.cfg_vpt > 0 ? a.cfg_vpt : spmv_vpt_choice(nv, (int)sizeof(T))),
                  (long long)A.n);
         tok = kt.begin(cls, spmv_launch_bytes<T>(A, vp, epi, nd, nv, a));

@@ -783,6 +783,10 @@ class ContinuedRefine : public Solver<double> {
     double theta = 0.0;
     double gap2 = 100.0;
     int max_inner = 1000;
+    // the solution correction of each iteration accumulates in FP32 and is added to the
+    // FP64 solution before each FP64 residual (key correction_fp32), instead of an FP64
+    // update of the solution every iteration
+    bool corr32 = false;
 
     float *v32(int k) { return b32[k].as<float>(); }
 
@@ -801,14 +805,17 @@ class ContinuedRefine : public Solver<double> {
         theta = prm.has("restart_threshold") ? prm.get_float("restart_threshold") : 0.0;
         const double gap = prm.has("restart_gap") ? prm.get_float("restart_gap") : 10.0;
         gap2 = gap * gap;
+        corr32 = prm.has("correction_fp32") && prm.get_bool("correction_fp32");
         if (!conv_check)
             conv_check = 1; // convergence is always decided on the FP64 residual
         hist_cap = std::max(2, max_inner + 2);
         histbuf.alloc(sizeof(double) * hist_cap * nv);
         b32.clear();
-        for (int i = 0; i < (pre ? 8 : 6); i++)
+        // r0, r, p, v, s, t, the FP32 solution correction, [p^, s^]
+        const int nbuf = pre ? 9 : 7;
+        for (int i = 0; i < nbuf; i++)
             b32.emplace_back(sizeof(float) * std::max<int64_t>(A->n, 1) * nv);
-        for (int i = 0; i < (pre ? 8 : 6); i++)
+        for (int i = 0; i < nbuf; i++)
             fill<float>(v32(i), A->n, nv, 0.0f);
         st32.alloc(sizeof(KrylovScalars<float>) + 256);
         k32 = st32.as<KrylovScalars<float>>();
@@ -858,8 +865,28 @@ class ContinuedRefine : public Solver<double> {
         if (xzero)
             fill<double>(x, A->n, nv, 0.0);
         float *r0 = v32(0), *r = v32(1), *p = v32(2), *v = v32(3), *s = v32(4), *t = v32(5);
-        float *p_ = pre ? v32(6) : p, *s_ = pre ? v32(7) : s;
+        float *xc = v32(6);
+        float *p_ = pre ? v32(7) : p, *s_ = pre ? v32(8) : s;
         const int *H = &cs->halt;
+        if (corr32)
+            fill<float>(xc, A->n, nv, 0.0f);
+        // x(FP64) += xc and xc = 0, while *gate (before every FP64 residual of x)
+        auto fold = [&](const int *gate, const int *halt) {
+            if (!corr32)
+                return;
+            RedArgs ra = red_args(halt);
+            ra.gate = gate;
+            EwArgs<double> a;
+            a.n = A->n;
+            a.z = x;
+            a.xo = xc;
+            ew<double>(EW_ADD_CONV, 0, nv, a, ra, FinArgs<double>{});
+            EwArgs<float> z;
+            z.n = A->n;
+            z.z = xc;
+            z.c0 = 0.0f;
+            ew<float>(EW_FILL, 0, nv, z, ra, FinArgs<float>{});
+        };
         { // FP64 r = b - A x stored as FP32 r0, r, p; FP64 norm and initial check
             SpmvArgs<double> a{};
             a.x = x;
@@ -900,7 +927,18 @@ class ContinuedRefine : public Solver<double> {
                 a.w = s;
                 spmv<float>(*A, Prec::F32, EPI_AXY, 3, nv, a, red_args(H), fin32(FIN_MX_OMEGA));
             }
-            { // x(FP64) += alpha_c p_ + omega_c s_
+            if (corr32) { // FP32 correction xc += alpha_c p_ + omega_c s_, folded into
+                          // the FP64 x before each FP64 residual (group-wise updates)
+                EwArgs<float> a;
+                a.n = A->n;
+                a.z = xc;
+                a.x = p_;
+                a.y = s_;
+                a.a = k32->ac;
+                a.b = k32->oc;
+                a.freeze_zero = 1;
+                ew<float>(EW_AXPBYPCZ, nv, a, H);
+            } else { // x(FP64) += alpha_c p_ + omega_c s_
                 EwArgs<double> a;
                 a.n = A->n;
                 a.z = x;
@@ -911,6 +949,7 @@ class ContinuedRefine : public Solver<double> {
                 a.freeze_zero = 1;
                 ew<double>(EW_MX_XUPD, nv, a, H);
             }
+            fold(&cs->replace, H);
             { // FP64 residual replacement when requested
                 SpmvArgs<double> a{};
                 a.x = x;
@@ -940,6 +979,7 @@ class ContinuedRefine : public Solver<double> {
                 ew<float>(EW_AXPBY_Z, theta > 0 ? 3 : 1, nv, a, ra, fa);
             }
             if (theta > 0) { // (r, r0) lost: FP64 residual, r0 = p = r
+                fold(&cs->rstnow, H);
                 SpmvArgs<double> a{};
                 a.x = x;
                 a.b = b;
@@ -964,6 +1004,7 @@ class ContinuedRefine : public Solver<double> {
             }
         };
         run_loop(max_inner, cs, iteration);
+        fold(nullptr, nullptr); // corrections since the last FP64 check (iteration limit)
     }
     void stats(SolveStats &st, bool with_hist) override {
         Solver<double>::stats(st, with_hist);

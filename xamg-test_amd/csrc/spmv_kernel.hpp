@@ -32,9 +32,12 @@ __device__ __forceinline__ int logical_block(int b, int G, int remap) {
     return b;
 }
 
-
+// Workgroup partial sums of the fused dots: lanes, then waves, in a fixed order; the
+// partial of logical block L (the L-th row range) goes to slot L, so the finish kernel
+// combines the row ranges in row order whatever the XCD mapping.
 template <typename T, int NV, int ND, int VPT>
-__device__ void reduce_and_finish(double (&acc)[ND][VPT], const RedArgs &r, const FinArgs<T> &f) {
+__device__ void reduce_and_finish(double (&acc)[ND][VPT], const RedArgs &r, const FinArgs<T> &f,
+                                  int L) {
     constexpr int LR = NV / VPT;
     __shared__ double lds[kBlock / 64][ND][NV];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -58,29 +61,54 @@ __device__ void reduce_and_finish(double (&acc)[ND][VPT], const RedArgs &r, cons
 #pragma unroll
         for (int w = 1; w < kBlock / 64; w++)
             s += lds[w][k][l];
-        r.ws.partials[(int64_t)tid * r.ws.max_blocks + blockIdx.x] = s;
+        r.ws.partials[(int64_t)tid * r.ws.max_blocks + L] = s;
     }
     (void)G;
     (void)f;
 }
 
-// Loads / stores of VPT consecutive vector elements (16-byte accesses where possible).
-template <typename T, int VPT>
+// Non-temporal (streaming) load of an operand read once per launch, so that it does not
+// displace the gathered x from L2.
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ __hip_bfloat16 ld_nt<__hip_bfloat16>(const __hip_bfloat16 *p) {
+    const unsigned short u = __builtin_nontemporal_load((const unsigned short *)p);
+    __hip_bfloat16 r;
+    __builtin_memcpy(&r, &u, 2);
+    return r;
+}
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+// Loads / stores of VPT consecutive vector elements (16-byte accesses where possible);
+// NT: non-temporal loads.
+template <typename T, int VPT, bool NT = false>
 __device__ __forceinline__ void ld_vec(const T *__restrict__ p, T (&v)[VPT]) {
     if constexpr (sizeof(T) * VPT % 16 == 0) {
-        using V4 = uint4;
 #pragma unroll
         for (int q = 0; q < (int)(sizeof(T) * VPT / 16); q++) {
-            V4 t = ((const V4 *)p)[q];
+            u32x4_t t;
+            if constexpr (NT)
+                t = __builtin_nontemporal_load((const u32x4_t *)p + q);
+            else
+                t = ((const u32x4_t *)p)[q];
             __builtin_memcpy(&v[q * 16 / sizeof(T)], &t, 16);
         }
     } else if constexpr (sizeof(T) * VPT == 8) {
-        uint2 t = *(const uint2 *)p;
+        u32x2_t t;
+        if constexpr (NT)
+            t = __builtin_nontemporal_load((const u32x2_t *)p);
+        else
+            t = *(const u32x2_t *)p;
         __builtin_memcpy(&v[0], &t, 8);
     } else {
 #pragma unroll
         for (int q = 0; q < VPT; q++)
-            v[q] = p[q];
+            v[q] = NT ? ld_nt(p + q) : p[q];
     }
 }
 
@@ -117,7 +145,175 @@ __device__ __forceinline__ void st_vec(T *__restrict__ p, const T (&v)[VPT]) {
 // lanes of one row would otherwise read 4-byte entries a row length apart, so one
 // wavefront load touches one cache line per few lanes.  A tile whose nonzeros exceed
 // the LDS capacity is read directly, as with STAGE = 0.
+//
+// STAGE = 2: as STAGE = 1 with the operands that are read once per launch (column
+// indices, values, row operands of the epilogue and the dots) loaded non-temporally.
+// On a 7-point 128^3 grid the gathered x is reused 16384 rows later (the next plane),
+// which the L2 of an XCD only holds while the streamed operands do not evict it.
 // ---------------------------------------------------------------------------
+// One row of the SpMV family: the lanes of row i (VPT right-hand sides each, from l0)
+// walk nonzeros j0..j1 in chunks of U (all column indices, values and x segments of a
+// chunk loaded first, then added in ascending column order, csr_matvec.inl:213-231),
+// apply the epilogue, store, and add the row's terms to the fused dots.  nz(q, c, v)
+// fetches column index and value of entry q (from global memory or an LDS copy).
+template <typename V, typename T, int NV, int VPT, int EPI, int ND, bool NT, typename NZ>
+__device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0, uint32_t j0,
+                                         uint32_t j1, NZ &&nz,
+                                         double (&dacc)[ND > 0 ? ND : 1][VPT]) {
+    constexpr int U = (16 / VPT) > 2 ? (16 / VPT) : 2;
+    const int64_t e = i * NV + l0;
+    T acc[VPT];
+    if constexpr (EPI == EPI_AXPY) {
+        ld_vec<T, VPT, NT>(a.y + e, acc);
+    } else {
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            acc[v] = (T)0;
+    }
+    // the epilogue's own row operands are loaded before the gathers so their
+    // latency overlaps the nonzero loop
+    constexpr bool kNeedB = EPI != EPI_AXY && EPI != EPI_AXPY;
+    constexpr bool kNeedD = EPI == EPI_JACOBI || EPI == EPI_CHEB || EPI == EPI_CHEB_LAST;
+    T pb[VPT], pw[VPT], px[VPT];
+    T pdi = (T)0;
+    if constexpr (kNeedB)
+        ld_vec<T, VPT, NT>(a.b + e, pb);
+    if constexpr (kNeedD)
+        pdi = a.d[i];
+    if constexpr (EPI == EPI_JACOBI) {
+        ld_vec<T, VPT, NT>(a.xin + e, px);
+    } else if constexpr (EPI == EPI_CHEB_LAST) {
+        if (a.xin != nullptr)
+            ld_vec<T, VPT, NT>(a.xin + e, px);
+    }
+    if constexpr (ND == 3) {
+        ld_vec<T, VPT, NT>(a.w + e, pw);
+    } else if constexpr (ND == 1) {
+        if (a.w != nullptr)
+            ld_vec<T, VPT, NT>(a.w + e, pw);
+    }
+    for (uint32_t j = j0; j < j1; j += U) {
+        uint32_t c[U];
+        T av[U];
+        T xv[U][VPT];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool ok = j + u < j1;
+            T v;
+            nz(ok ? j + u : j0, c[u], v);
+            av[u] = ok ? v : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            ld_vec<T, VPT>(a.x + (int64_t)c[u] * NV + l0, xv[u]);
+        if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
+            if (a.x_is_rhs && a.prescaled) {
+                // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const T dc = a.dcol[c[u]];
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        xv[u][v] = a.xcoef * (xv[u][v] * dc);
+                }
+            } else if (a.x_is_rhs) {
+                // first Horner step read straight from the right-hand side:
+                // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const T dc = a.dcol[c[u]];
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        xv[u][v] = (a.xcoef * (xv[u][v] * dc)) * dc;
+                }
+            } else if (!a.prescaled) {
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const T dc = a.dcol[c[u]];
+#pragma unroll
+                    for (int v = 0; v < VPT; v++)
+                        xv[u][v] = xv[u][v] * dc;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (j + u < j1)
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    acc[v] = acc[v] + xv[u][v] * av[u];
+    }
+    T out[VPT];
+    if constexpr (EPI == EPI_AXY || EPI == EPI_AXPY) {
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            out[v] = acc[v];
+    } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            out[v] = (T)1.0 * pb[v] + (T)(-1.0) * acc[v];
+    } else if constexpr (EPI == EPI_JACOBI) {
+        const T di = pdi;
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            out[v] = px[v] + (a.wcoef[l0 + v] * di) * (pb[v] - acc[v]);
+    } else {
+        const T di = pdi;
+#pragma unroll
+        for (int v = 0; v < VPT; v++) {
+            const T rr = pb[v] * di;
+            const T t2 = a.prescaled ? acc[v] : acc[v] * di;
+            out[v] = a.coef * rr + (T)1.0 * t2;
+        }
+        if constexpr (EPI == EPI_CHEB_LAST) {
+#pragma unroll
+            for (int v = 0; v < VPT; v++)
+                out[v] = out[v] * di;
+            if (a.xin != nullptr) {
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    out[v] = (T)1.0 * out[v] + (T)1.0 * px[v];
+            }
+        }
+    }
+    if constexpr (EPI == EPI_RESID_MX) {
+        float o32[VPT];
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            o32[v] = (float)out[v];
+        st_vec<float, VPT>(a.yo + e, o32);
+        if (a.yo2 != nullptr)
+            st_vec<float, VPT>(a.yo2 + e, o32);
+        if (a.yo3 != nullptr)
+            st_vec<float, VPT>(a.yo3 + e, o32);
+    } else {
+        st_vec<T, VPT>(a.y + e, out);
+        if (a.y2 != nullptr)
+            st_vec<T, VPT>(a.y2 + e, out);
+        if (a.y3 != nullptr)
+            st_vec<T, VPT>(a.y3 + e, out);
+    }
+    if constexpr (ND == 1) {
+        if (a.w != nullptr) {
+#pragma unroll
+            for (int v = 0; v < VPT; v++)
+                dacc[0][v] += (double)out[v] * (double)pw[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < VPT; v++)
+                dacc[0][v] += (double)out[v] * (double)out[v];
+        }
+    } else if constexpr (ND == 3) {
+#pragma unroll
+        for (int v = 0; v < VPT; v++) {
+            const double o = (double)out[v], wv = (double)pw[v];
+            dacc[0][v] += o * wv;
+            dacc[1][v] += o * o;
+            dacc[2][v] += wv * wv;
+        }
+    }
+}
+
 constexpr int kStageCap = 2048; // staged nonzeros per tile (16 per row at 128 rows)
 
 template <typename V, typename T, int NV, int VPT, int EPI, int ND, int STAGE>
@@ -129,11 +325,9 @@ __global__ void __launch_bounds__(kBlock)
         return;
     constexpr int LR = NV / VPT;
     constexpr int RPB = kBlock / LR;
-    constexpr int U = (16 / VPT) > 2 ? (16 / VPT) : 2;
     const V *__restrict__ val = (const V *)a.val;
     const uint32_t *__restrict__ row = a.row;
     const uint32_t *__restrict__ col = a.col;
-    const T *__restrict__ x = a.x;
     const int tid = threadIdx.x, lr = tid / LR, sub = tid % LR;
     const int l0 = sub * VPT;
     const int G = gridDim.x;
@@ -147,6 +341,7 @@ __global__ void __launch_bounds__(kBlock)
         for (int v = 0; v < VPT; v++)
             dacc[k][v] = 0.0;
     constexpr int CAP = STAGE ? kStageCap : 1;
+    constexpr bool NT = STAGE == 2;
     __shared__ uint32_t s_col[CAP];
     __shared__ V s_val[CAP];
     for (int64_t base = r0; base < r1; base += RPB) {
@@ -161,177 +356,137 @@ __global__ void __launch_bounds__(kBlock)
             __syncthreads(); // the previous tile's rows are done with the LDS copy
             if (staged)
                 for (uint32_t k = tid; k < jB - jA; k += kBlock) {
-                    s_col[k] = col[jA + k];
-                    s_val[k] = val[jA + k];
+                    s_col[k] = NT ? ld_nt(col + jA + k) : col[jA + k];
+                    s_val[k] = NT ? ld_nt(val + jA + k) : val[jA + k];
                 }
             __syncthreads();
         }
         if (i < r1) {
-            const int64_t e = i * NV + l0;
             const uint32_t j0 = row[i], j1 = row[i + 1];
-            T acc[VPT];
-            if constexpr (EPI == EPI_AXPY) {
-                ld_vec<T, VPT>(a.y + e, acc);
-            } else {
-#pragma unroll
-                for (int v = 0; v < VPT; v++)
-                    acc[v] = (T)0;
-            }
-            // the epilogue's own row operands are loaded before the gathers so their
-            // latency overlaps the nonzero loop
-            constexpr bool kNeedB = EPI != EPI_AXY && EPI != EPI_AXPY;
-            constexpr bool kNeedD = EPI == EPI_JACOBI || EPI == EPI_CHEB || EPI == EPI_CHEB_LAST;
-            T pb[VPT], pw[VPT], px[VPT];
-            T pdi = (T)0;
-            if constexpr (kNeedB)
-                ld_vec<T, VPT>(a.b + e, pb);
-            if constexpr (kNeedD)
-                pdi = a.d[i];
-            if constexpr (EPI == EPI_JACOBI) {
-                ld_vec<T, VPT>(a.xin + e, px);
-            } else if constexpr (EPI == EPI_CHEB_LAST) {
-                if (a.xin != nullptr)
-                    ld_vec<T, VPT>(a.xin + e, px);
-            }
-            if constexpr (ND == 3) {
-                ld_vec<T, VPT>(a.w + e, pw);
-            } else if constexpr (ND == 1) {
-                if (a.w != nullptr)
-                    ld_vec<T, VPT>(a.w + e, pw);
-            }
-            for (uint32_t j = j0; j < j1; j += U) {
-                uint32_t c[U];
-                T av[U];
-                T xv[U][VPT];
-                if (STAGE && staged) {
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const bool ok = j + u < j1;
-                        const uint32_t q = (ok ? j + u : j0) - jb;
-                        c[u] = s_col[q];
-                        av[u] = ok ? to_t<T>(s_val[q]) : (T)0;
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const bool ok = j + u < j1;
-                        c[u] = ok ? col[j + u] : col[j0];
-                        av[u] = ok ? to_t<T>(val[j + u]) : (T)0;
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++)
-                    ld_vec<T, VPT>(x + (int64_t)c[u] * NV + l0, xv[u]);
-                if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
-                    if (a.x_is_rhs && a.prescaled) {
-                        // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
-#pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const T dc = a.dcol[c[u]];
-#pragma unroll
-                            for (int v = 0; v < VPT; v++)
-                                xv[u][v] = a.xcoef * (xv[u][v] * dc);
-                        }
-                    } else if (a.x_is_rhs) {
-                        // first Horner step read straight from the right-hand side:
-                        // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
-#pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const T dc = a.dcol[c[u]];
-#pragma unroll
-                            for (int v = 0; v < VPT; v++)
-                                xv[u][v] = (a.xcoef * (xv[u][v] * dc)) * dc;
-                        }
-                    } else if (!a.prescaled) {
-#pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const T dc = a.dcol[c[u]];
-#pragma unroll
-                            for (int v = 0; v < VPT; v++)
-                                xv[u][v] = xv[u][v] * dc;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++)
-                    if (j + u < j1)
-#pragma unroll
-                        for (int v = 0; v < VPT; v++)
-                            acc[v] = acc[v] + xv[u][v] * av[u];
-            }
-            T out[VPT];
-            if constexpr (EPI == EPI_AXY || EPI == EPI_AXPY) {
-#pragma unroll
-                for (int v = 0; v < VPT; v++)
-                    out[v] = acc[v];
-            } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
-#pragma unroll
-                for (int v = 0; v < VPT; v++)
-                    out[v] = (T)1.0 * pb[v] + (T)(-1.0) * acc[v];
-            } else if constexpr (EPI == EPI_JACOBI) {
-                const T di = pdi;
-#pragma unroll
-                for (int v = 0; v < VPT; v++)
-                    out[v] = px[v] + (a.wcoef[l0 + v] * di) * (pb[v] - acc[v]);
-            } else {
-                const T di = pdi;
-#pragma unroll
-                for (int v = 0; v < VPT; v++) {
-                    const T rr = pb[v] * di;
-                    const T t2 = a.prescaled ? acc[v] : acc[v] * di;
-                    out[v] = a.coef * rr + (T)1.0 * t2;
-                }
-                if constexpr (EPI == EPI_CHEB_LAST) {
-#pragma unroll
-                    for (int v = 0; v < VPT; v++)
-                        out[v] = out[v] * di;
-                    if (a.xin != nullptr) {
-#pragma unroll
-                        for (int v = 0; v < VPT; v++)
-                            out[v] = (T)1.0 * out[v] + (T)1.0 * px[v];
-                    }
-                }
-            }
-            if constexpr (EPI == EPI_RESID_MX) {
-                float o32[VPT];
-#pragma unroll
-                for (int v = 0; v < VPT; v++)
-                    o32[v] = (float)out[v];
-                st_vec<float, VPT>(a.yo + e, o32);
-                if (a.yo2 != nullptr)
-                    st_vec<float, VPT>(a.yo2 + e, o32);
-                if (a.yo3 != nullptr)
-                    st_vec<float, VPT>(a.yo3 + e, o32);
-            } else {
-                st_vec<T, VPT>(a.y + e, out);
-                if (a.y2 != nullptr)
-                    st_vec<T, VPT>(a.y2 + e, out);
-                if (a.y3 != nullptr)
-                    st_vec<T, VPT>(a.y3 + e, out);
-            }
-            if constexpr (ND == 1) {
-                if (a.w != nullptr) {
-#pragma unroll
-                    for (int v = 0; v < VPT; v++)
-                        dacc[0][v] += (double)out[v] * (double)pw[v];
-                } else {
-#pragma unroll
-                    for (int v = 0; v < VPT; v++)
-                        dacc[0][v] += (double)out[v] * (double)out[v];
-                }
-            } else if constexpr (ND == 3) {
-#pragma unroll
-                for (int v = 0; v < VPT; v++) {
-                    const double o = (double)out[v], wv = (double)pw[v];
-                    dacc[0][v] += o * wv;
-                    dacc[1][v] += o * o;
-                    dacc[2][v] += wv * wv;
-                }
-            }
+            spmv_row<V, T, NV, VPT, EPI, ND, NT>(a, i, l0, j0, j1,
+                                                 [&](uint32_t q, uint32_t &c, T &v) {
+                                                     if (STAGE && staged) {
+                                                         c = s_col[q - jb];
+                                                         v = to_t<T>(s_val[q - jb]);
+                                                     } else {
+                                                         c = NT ? ld_nt(col + q) : col[q];
+                                                         v = to_t<T>(NT ? ld_nt(val + q) : val[q]);
+                                                     }
+                                                 },
+                                                 dacc);
         }
     }
     if constexpr (ND > 0)
-        reduce_and_finish<T, NV, ND, VPT>(dacc, r, f);
+        reduce_and_finish<T, NV, ND, VPT>(dacc, r, f, L);
+}
+
+// STAGE = 3 (values of 4 or 8 bytes): tiles of RPB rows staged through LDS with a
+// double buffer filled by the direct global-to-LDS loads of gfx950 (global_load_lds:
+// each lane's dword goes to LDS without passing through registers).  While the rows of
+// tile t gather x, the row pointers, column indices and values of tile t+1 are already
+// on their way into the other buffer, so a tile costs one dependent round trip to
+// memory (the x gathers) instead of three (row pointers, column indices, x).  Tile t+1
+// starts at tile t's last row pointer, so one scalar row-pointer load, issued a tile
+// ahead, bounds each copy.  A tile with more than CAP nonzeros reads them directly.
+constexpr int kPipeNnzPerRow = 8;
+
+template <typename V, typename T, int NV, int VPT, int EPI, int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_spmv_pipe(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
+    static_assert(sizeof(V) % 4 == 0, "values are copied as dwords");
+    if (r.halt != nullptr && *r.halt)
+        return;
+    if (r.gate != nullptr && !*r.gate)
+        return;
+    constexpr int LR = NV / VPT;
+    constexpr int RPB = kBlock / LR;
+    constexpr int CAP = RPB * kPipeNnzPerRow;
+    constexpr int VW = (int)(sizeof(V) / 4);
+    const V *__restrict__ val = (const V *)a.val;
+    const uint32_t *__restrict__ row = a.row;
+    const uint32_t *__restrict__ col = a.col;
+    const int tid = threadIdx.x, lr = tid / LR, sub = tid % LR;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int l0 = sub * VPT;
+    const int G = gridDim.x;
+    const int L = logical_block(blockIdx.x, G, remap);
+    const int64_t r0 = a.n * L / G, r1 = a.n * (L + 1) / G;
+    constexpr int NDX = ND > 0 ? ND : 1;
+    double dacc[NDX][VPT];
+#pragma unroll
+    for (int k = 0; k < NDX; k++)
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            dacc[k][v] = 0.0;
+    __shared__ uint32_t s_row[2][RPB + 1];
+    __shared__ uint32_t s_col[2][CAP];
+    __shared__ __align__(16) uint32_t s_vw[2][CAP * VW];
+    // copy of tile rows [base, bend] (row pointers) and its nonzeros [jA, jB) into buffer b
+    auto issue = [&](int b, int64_t base, int64_t bend, uint32_t jA, uint32_t jB) {
+        const uint32_t nr = (uint32_t)(bend - base) + 1;
+        for (uint32_t k0 = wave * 64; k0 < nr; k0 += kBlock)
+            if (k0 + lane < nr)
+                __builtin_amdgcn_global_load_lds(row + base + k0 + lane, &s_row[b][k0], 4, 0, 0);
+        const uint32_t cnt = jB - jA;
+        if (cnt <= (uint32_t)CAP) {
+            for (uint32_t k0 = wave * 64; k0 < cnt; k0 += kBlock)
+                if (k0 + lane < cnt)
+                    __builtin_amdgcn_global_load_lds(col + jA + k0 + lane, &s_col[b][k0], 4, 0, 0);
+            const uint32_t *vw = (const uint32_t *)val + (size_t)jA * VW;
+            for (uint32_t k0 = wave * 64; k0 < cnt * VW; k0 += kBlock)
+                if (k0 + lane < cnt * VW)
+                    __builtin_amdgcn_global_load_lds(vw + k0 + lane, &s_vw[b][k0], 4, 0, 0);
+        }
+    };
+    const int ntiles = (int)((r1 - r0 + RPB - 1) / RPB);
+    int64_t base = r0, bend = r0 + RPB < r1 ? r0 + RPB : r1;
+    uint32_t jA = 0, jB = 0, jN = 0;
+    if (ntiles > 0) {
+        jA = row[base];
+        jB = row[bend];
+        issue(0, base, bend, jA, jB);
+        if (bend < r1)
+            jN = row[bend + RPB < r1 ? bend + RPB : r1];
+    }
+    for (int t = 0; t < ntiles; t++) {
+        const int b = t & 1;
+        // every lane's copies of tile t have landed and every row of tile t-1 is done
+        // with buffer b^1
+        __syncthreads();
+        const int64_t nbase = bend, nbend = nbase + RPB < r1 ? nbase + RPB : r1;
+        uint32_t jN2 = 0;
+        if (t + 1 < ntiles) {
+            issue(b ^ 1, nbase, nbend, jB, jN);
+            if (nbend < r1)
+                jN2 = row[nbend + RPB < r1 ? nbend + RPB : r1];
+        }
+        const int64_t i = base + lr;
+        if (i < bend) {
+            const uint32_t j0 = s_row[b][lr], j1 = s_row[b][lr + 1];
+            const bool staged = jB - jA <= (uint32_t)CAP;
+            spmv_row<V, T, NV, VPT, EPI, ND, false>(a, i, l0, j0, j1,
+                                                    [&](uint32_t q, uint32_t &c, T &v) {
+                                                        if (staged) {
+                                                            c = s_col[b][q - jA];
+                                                            V w;
+                                                            __builtin_memcpy(&w, &s_vw[b][(q - jA) * VW],
+                                                                             sizeof(V));
+                                                            v = to_t<T>(w);
+                                                        } else {
+                                                            c = col[q];
+                                                            v = to_t<T>(val[q]);
+                                                        }
+                                                    },
+                                                    dacc);
+        }
+        base = nbase;
+        bend = nbend;
+        jA = jB;
+        jB = jN;
+        jN = jN2;
+    }
+    if constexpr (ND > 0)
+        reduce_and_finish<T, NV, ND, VPT>(dacc, r, f, L);
 }
 
 template <typename V, typename T, int NV, int VPT, int STAGE>
@@ -354,7 +509,11 @@ void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const Fin
     dim3 g(G), b(kBlock);
 #define XAMG_SPMV_CASE(E, D)                                                                       \
     if (epi == E && nd == D) {                                                                     \
-        hipLaunchKernelGGL((k_spmv<V, T, NV, VPT, E, D, STAGE>), g, b, 0, s, a, r, f, c.xcd_remap);       \
+        if constexpr (STAGE == 3 && sizeof(V) % 4 == 0)                                            \
+            hipLaunchKernelGGL((k_spmv_pipe<V, T, NV, VPT, E, D>), g, b, 0, s, a, r, f, c.xcd_remap); \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_spmv<V, T, NV, VPT, E, D, (STAGE == 3 ? 1 : STAGE)>), g, b, 0, s, a, \
+                               r, f, c.xcd_remap);                                                  \
         launch_finish<T, NV>(D, r, f, G, s);                                                       \
         return;                                                                                    \
     }
@@ -409,7 +568,9 @@ void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinA
 }
 
 // Launch of the SpMV family for one staging mode (0: column indices and values read by
-// each row's lanes, 1: staged through LDS per tile of rows).
+// each row's lanes, 1: staged through LDS per tile of rows, 2: as 1 with non-temporal
+// streamed operands, 3: double-buffered LDS staging by direct global-to-LDS loads; BF16
+// values take mode 1 there).
 template <typename V, typename T, int STAGE>
 void launch_spmv_mode(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                       const FinArgs<T> &f, hipStream_t s) {

@@ -1,5 +1,5 @@
 // The LDS-staged instantiations of the SpMV family (STAGE = 1), compiled apart from
-// kernels.hip so the two sets build in parallel.
+// kernels.hip so the sets build in parallel.
 #include "spmv_kernel.hpp"
 
 namespace xamg {
@@ -7,7 +7,10 @@ namespace xamg {
 template <typename V, typename T>
 void launch_spmv_staged(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                         const FinArgs<T> &f, hipStream_t s) {
-    launch_spmv_mode<V, T, 1>(epi, ndots, nv, a, r, f, s);
+    if (a.cfg_mode == 2)
+        launch_spmv_stage2<V, T>(epi, ndots, nv, a, r, f, s);
+    else
+        launch_spmv_mode<V, T, 1>(epi, ndots, nv, a, r, f, s);
 }
 
 template void launch_spmv_staged<double, double>(int, int, int, const SpmvArgs<double> &,

@@ -579,7 +579,8 @@ def set_spmv_vpt(nv, es, vpt):
 
 
 def set_spmv_stage(mode):
-    """LDS staging in every SpMV: -1 as autotuned, 0 off, 1 on."""
+    """SpMV staging mode of every launch: -1 as autotuned, 0, 1 (LDS) or 2 (LDS +
+    non-temporal streamed operands)."""
     check(lib().xamgx_set_spmv_stage(mode))
 
 
