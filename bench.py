@@ -375,9 +375,9 @@ def main():
     if os.path.exists(prof):
         cls = json.load(open(prof)).get("classes", {})
         rec = cls.get(dom["name"])
-        if rec is None:  # same kernel under another RHS-per-lane choice of the autotuner
-            base = re.sub(r" vpt\d+", "", dom["name"])
-            rec = next((v for k, v in cls.items() if re.sub(r" vpt\d+", "", k) == base), None)
+        if rec is None:  # the same epilogue under another launch choice of the autotuner
+            strip = lambda k: re.sub(r" (stage|vpt)\d+", "", k)
+            rec = next((v for k, v in cls.items() if strip(k) == strip(dom["name"])), None)
         if rec is not None:
             traffic = rec["traffic_bytes_per_launch"]
 
