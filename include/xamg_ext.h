@@ -95,7 +95,9 @@ int xamgx_set_spmv_vpt(int nv, int es, int vpt);
 /* Staging mode of every SpMV launch: -1 as autotuned (default), 0 direct loads, 1 each
  * row tile's column indices and values staged through LDS, 2 as 1 with the streamed
  * operands loaded non-temporally, 3 double-buffered staging by direct global-to-LDS
- * loads (BF16 values: as 1).  Results are identical in every mode. */
+ * loads (BF16 values: as 1), 4 and 5 as 2 and 3 with the column indices read as 16-bit
+ * offsets from the row (matrices whose entries all lie within 32767 of the diagonal;
+ * otherwise as 2 and 3).  Results are identical in every mode. */
 int xamgx_set_spmv_stage(int mode);
 /* Spectrum bounds of D^{-1/2} A D^{-1/2} used by Chebyshev (the reference takes
  * them from hypre's setup, hypre_base.inl:369-375). */

@@ -1,8 +1,10 @@
 """The LDS-staged SpMV modes (1: column indices and values of each row tile copied to
 LDS first; 2: the same with non-temporal loads of the streamed operands; 3: double-
-buffered staging by direct global-to-LDS loads) against the oracle, bit for bit, at
-every compiled RHS-per-lane width; tiles whose nonzeros exceed the LDS capacity take the
-direct path inside the same launch; a solve gives identical iterates in every mode."""
+buffered staging by direct global-to-LDS loads; 4, 5: modes 2, 3 with 16-bit column
+offsets) against the oracle, bit for bit, at every compiled RHS-per-lane width; tiles
+whose nonzeros exceed the LDS capacity take the direct path inside the same launch; a
+matrix too wide for 16-bit offsets runs modes 4, 5 as 2, 3; a solve gives identical
+iterates in every mode."""
 import numpy as np
 import pytest
 
@@ -14,7 +16,7 @@ pytestmark = pytest.mark.gpu
 VPTS = {1: [1], 2: [1, 2], 4: [2, 4], 8: [2, 4, 8], 16: [4, 8, 16]}
 
 
-@pytest.fixture(params=[1, 2, 3])
+@pytest.fixture(params=[1, 2, 3, 4, 5])
 def staged(xamg, request):
     xamg.set_spmv_stage(request.param)
     yield xamg
@@ -102,7 +104,7 @@ def test_staged_solve_identical_iterates(xamg):
                                  "spectrum_fraction": 0.3}}
     outs = []
     try:
-        for mode in (0, 1, 2, 3):
+        for mode in (0, 1, 2, 3, 4, 5):
             xamg.set_spmv_stage(mode)
             outs.append(xamg.solve_system(row, col, val, b, nv, params, eigs=(1 - c, 1 + c)))
     finally:
@@ -112,3 +114,26 @@ def test_staged_solve_identical_iterates(xamg):
         assert a["iters"] == s["iters"]
         assert np.array_equal(a["x"].view(np.uint64), s["x"].view(np.uint64))
         assert np.array_equal(a["hist"], s["hist"])
+
+
+def test_staged_wide_matrix_falls_back(staged):
+    # entries 40000 rows off the diagonal: no 16-bit offsets, modes 4/5 run as 2/3
+    X = staged
+    n = 70000
+    rows, cols = [], []
+    for i in range(n):
+        c = sorted({max(i - 1, 0), i, min(i + 1, n - 1), (i + 40000) % n})
+        rows.append(len(c))
+        cols.extend(c)
+    row = np.zeros(n + 1, dtype=np.int64)
+    row[1:] = np.cumsum(rows)
+    col = np.array(cols, dtype=np.int32)
+    rng = np.random.default_rng(13)
+    val = rng.standard_normal(len(col))
+    A = X.Matrix(row.astype(np.int32), col, val)
+    nv = 4
+    x = rng.standard_normal(n * nv)
+    ref = O.spmv(O.csr(row, col, val), x, nv)
+    y = A.spmv(x, nv)
+    assert np.array_equal(y.view(np.uint64), ref.view(np.uint64))
+    A.destroy()

@@ -3,7 +3,7 @@
 // 64 * VPT / nv and the nonzero unroll 16 / VPT), the workgroup counts, the XCD-aware row
 // mapping and the staging modes (LDS staging of column indices and values, non-temporal
 // loads of the streamed operands, double-buffered staging by direct global-to-LDS
-// loads) on the matrix itself, and
+// loads, 16-bit column offsets for banded matrices) on the matrix itself, and
 // keeps the fastest.  Each candidate launch is timed cold: a 512 MiB copy before it
 // evicts the L2 caches and the 256 MB last-level cache, as the other vectors of a
 // solver iteration do between two SpMVs (timed back to back, the matrix and vectors of a
@@ -166,7 +166,12 @@ DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
     const std::vector<int> vpts = nd > 0 ? std::vector<int>{0} : vpt_options(nv);
     const std::vector<int> grids =
         nd > 0 ? std::vector<int>{launch_cfg().grid_red} : std::vector<int>{1024, 2048, 4096, 8192};
-    for (int mode : {0, 1, 2, 3})
+    std::vector<int> modes = {0, 1, 2, 3};
+    if (A.col16.p != nullptr) {
+        modes.push_back(4);
+        modes.push_back(5);
+    }
+    for (int mode : modes)
         for (int vpt : vpts)
             for (int grid : grids)
                 for (int remap : {0, 1}) {
@@ -199,7 +204,7 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
     if (!autotune_enabled() || n < 65536)
         return t;
     std::ostringstream k;
-    k << "cold5_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
+    k << "cold6_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
       << nv << "_epi" << epi << "_nd" << nd;
     const std::string key = k.str();
     auto &c = cache();

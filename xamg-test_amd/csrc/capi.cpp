@@ -928,7 +928,7 @@ int xamgx_kernel_stats_get(int i, char *name, int len, int64_t *calls, double *m
 
 int xamgx_set_spmv_stage(int mode) {
     return guard([&] {
-        XAMG_REQUIRE(mode >= -1 && mode <= 3, "mode must be -1 (tuned), 0, 1, 2 or 3");
+        XAMG_REQUIRE(mode >= -1 && mode <= 5, "mode must be -1 (tuned) or 0..5");
         launch_cfg().spmv_stage = mode;
     });
 }

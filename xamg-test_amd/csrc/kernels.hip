@@ -324,18 +324,30 @@ __global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, i
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int p = wave; p < ND * NV; p += kFinBlock / 64) {
         const double *src = r.ws.partials + (int64_t)p * r.ws.max_blocks;
-        // 8 independent loads in flight per lane; fixed combination order
+        // fixed combination order: q[u] sums partials lane + 64 (u + 8k) in increasing k
         double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int b0 = lane; b0 < G; b0 += 64 * 8) {
-            double v[8];
+        if (G <= 64 * 32 && r.ws.max_blocks >= 64 * 32) {
+            // up to 2048 partials: all 32 loads of a lane in flight at once, unconditional
+            // (the array holds max_blocks slots) and masked by a select, not a branch
+            double v[32];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int b = b0 + 64 * u;
-                v[u] = b < G ? src[b] : 0.0;
+            for (int u = 0; u < 32; u++)
+                v[u] = src[lane + 64 * u];
+#pragma unroll
+            for (int u = 0; u < 32; u++)
+                q[u & 7] += lane + 64 * u < G ? v[u] : 0.0;
+        } else {
+            for (int b0 = lane; b0 < G; b0 += 64 * 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int b = b0 + 64 * u;
+                    v[u] = b < G ? src[b] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    q[u] += v[u];
             }
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                q[u] += v[u];
         }
         double s = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
 #pragma unroll
@@ -611,8 +623,14 @@ int &spmv_vpt_choice(int nv, int es) {
 template <typename V, typename T>
 void launch_spmv(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                  const FinArgs<T> &f, hipStream_t s) {
-    if (a.cfg_mode == 3)
+    if (a.cfg_mode == 5 && a.col16 != nullptr)
+        launch_spmv_stage5<V, T>(epi, ndots, nv, a, r, f, s);
+    else if (a.cfg_mode == 4 && a.col16 != nullptr)
+        launch_spmv_stage4<V, T>(epi, ndots, nv, a, r, f, s);
+    else if (a.cfg_mode == 3 || a.cfg_mode == 5)
         launch_spmv_stage3<V, T>(epi, ndots, nv, a, r, f, s);
+    else if (a.cfg_mode == 4)
+        launch_spmv_stage2<V, T>(epi, ndots, nv, a, r, f, s);
     else if (a.cfg_mode == 1 || a.cfg_mode == 2)
         launch_spmv_staged<V, T>(epi, ndots, nv, a, r, f, s);
     else

@@ -101,6 +101,7 @@ template <typename T>
 struct SpmvArgs {
     const uint32_t *row;
     const uint32_t *col;
+    const int16_t *col16 = nullptr; // column - row (banded matrices), staging modes 4 and 5
     const void *val; // V-typed
     int64_t n;
     const T *x;
@@ -122,7 +123,7 @@ struct SpmvArgs {
     // launch configuration chosen by the autotuner (0 / -1: global defaults); cfg_mode 1
     // stages each row tile's column indices and values through LDS, 2 also loads the
     // streamed operands non-temporally, 3 double-buffers the staging with direct
-    // global-to-LDS loads
+    // global-to-LDS loads; 4 and 5 are 2 and 3 with 16-bit column deltas (col16)
     int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1, cfg_mode = 0;
 };
 
@@ -156,7 +157,7 @@ struct LaunchCfg {
     int grid_ew = 2048;
     int grid_red = 2048;
     int xcd_remap = 1;
-    int spmv_stage = -1; // SpMV staging mode for every launch: -1 as tuned, else 0..3
+    int spmv_stage = -1; // SpMV staging mode for every launch: -1 as tuned, else 0..5
 };
 
 LaunchCfg &launch_cfg();
@@ -177,7 +178,8 @@ void launch_ew(int op, int ndots, int nv, const EwArgs<T> &a, const RedArgs &r,
 
 // The LDS-staged SpMV instantiations (spmv_staged.hip: cfg_mode 1; spmv_stage2.hip:
 // cfg_mode 2, streamed operands loaded non-temporally; spmv_stage3.hip: cfg_mode 3,
-// double-buffered staging by direct global-to-LDS loads).
+// double-buffered staging by direct global-to-LDS loads; spmv_stage4.hip and
+// spmv_stage5.hip: modes 2 and 3 reading 16-bit column deltas).
 template <typename V, typename T>
 void launch_spmv_staged(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                         const FinArgs<T> &f, hipStream_t s);
@@ -186,6 +188,12 @@ void launch_spmv_stage2(int epi, int ndots, int nv, const SpmvArgs<T> &a, const 
                         const FinArgs<T> &f, hipStream_t s);
 template <typename V, typename T>
 void launch_spmv_stage3(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                        const FinArgs<T> &f, hipStream_t s);
+template <typename V, typename T>
+void launch_spmv_stage4(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                        const FinArgs<T> &f, hipStream_t s);
+template <typename V, typename T>
+void launch_spmv_stage5(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                         const FinArgs<T> &f, hipStream_t s);
 
 // Final pass of a fused reduction over G workgroup partials (nd dot products per RHS),

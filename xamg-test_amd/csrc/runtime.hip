@@ -63,6 +63,25 @@ void DevMatrix::upload() {
         XAMG_HIP_CHECK(hipMemcpy(col.p, hcol.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
         XAMG_HIP_CHECK(hipMemcpy(val64.p, hval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
     }
+    col16.release();
+    bool banded = nnz > 0;
+    for (int64_t i = 0; i < n && banded; i++)
+        for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++) {
+            const int64_t d = (int64_t)hcol[j] - i;
+            if (d < -32767 || d > 32767) {
+                banded = false;
+                break;
+            }
+        }
+    if (banded) {
+        // padded by one entry: staging mode 5 copies whole dwords
+        std::vector<int16_t> d16(nnz + 2, 0);
+        for (int64_t i = 0; i < n; i++)
+            for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
+                d16[j] = (int16_t)((int64_t)hcol[j] - i);
+        col16.alloc(sizeof(int16_t) * d16.size());
+        XAMG_HIP_CHECK(hipMemcpy(col16.p, d16.data(), sizeof(int16_t) * d16.size(), hipMemcpyHostToDevice));
+    }
     (void)rt;
 }
 
@@ -300,7 +319,8 @@ static double spmv_launch_bytes(const DevMatrix &A, Prec vp, int epi, int nd, in
                                 const SpmvArgs<T> &a) {
     const double sv = vp == Prec::F64 ? 8 : vp == Prec::BF16 ? 2 : 4, st = sizeof(T);
     const double E = (double)A.n * nv * st, Dv = (double)A.n * st;
-    double b = 4.0 * (A.n + 1) + A.nnz * (4.0 + sv) + (double)A.ncols * nv * st;
+    const double si = a.cfg_mode >= 4 ? 2.0 : 4.0; // column index bytes
+    double b = 4.0 * (A.n + 1) + A.nnz * (si + sv) + (double)A.ncols * nv * st;
     if (epi == EPI_RESID_MX)
         b += (double)A.n * nv * 4 * (1 + (a.yo2 != nullptr) + (a.yo3 != nullptr));
     else
@@ -340,6 +360,7 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
     hipStream_t s = Runtime::get().stream;
     a.row = A.row.as<uint32_t>();
     a.col = A.col.as<uint32_t>();
+    a.col16 = A.col16.as<int16_t>();
     a.n = A.n;
     if (a.cfg_vpt == 0 && a.cfg_grid == 0) {
         const DevMatrix::Tune &t = A.tuned(vp, (int)sizeof(T), nv, epi, nd);
@@ -350,6 +371,8 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
     }
     if (launch_cfg().spmv_stage >= 0)
         a.cfg_mode = launch_cfg().spmv_stage;
+    if (a.cfg_mode >= 4 && a.col16 == nullptr)
+        a.cfg_mode -= 2; // 16-bit column deltas need a banded matrix
     KernelTimer &kt = KernelTimer::get();
     int tok = -1;
     if (kt.on()) {

@@ -79,6 +79,9 @@ struct DevMatrix {
     std::vector<uint32_t> hrow, hcol;
     std::vector<double> hval;
     DBuf row, col, val64, val32, valbf, val32s;
+    // column - row as int16 when every entry lies within 32767 of the diagonal (banded
+    // matrices such as grid stencils), read by SpMV staging modes 4 and 5; null otherwise
+    DBuf col16;
     DBuf inv_diag64, inv_sqrt_diag64, inv_diag32, inv_sqrt_diag32;
     DBuf dense_inv64, dense_inv32; // Direct: column-major inverse (inverse.inl:108-168)
     double min_eig = std::numeric_limits<double>::quiet_NaN();

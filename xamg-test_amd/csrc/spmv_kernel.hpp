@@ -150,6 +150,10 @@ __device__ __forceinline__ void st_vec(T *__restrict__ p, const T (&v)[VPT]) {
 // indices, values, row operands of the epilogue and the dots) loaded non-temporally.
 // On a 7-point 128^3 grid the gathered x is reused 16384 rows later (the next plane),
 // which the L2 of an XCD only holds while the streamed operands do not evict it.
+//
+// STAGE = 4: as STAGE = 2 reading the column indices as 16-bit offsets from the row
+// (col16, kept for matrices whose entries all lie within 32767 of the diagonal): half
+// the index bytes.
 // ---------------------------------------------------------------------------
 // One row of the SpMV family: the lanes of row i (VPT right-hand sides each, from l0)
 // walk nonzeros j0..j1 in chunks of U (all column indices, values and x segments of a
@@ -341,9 +345,12 @@ __global__ void __launch_bounds__(kBlock)
         for (int v = 0; v < VPT; v++)
             dacc[k][v] = 0.0;
     constexpr int CAP = STAGE ? kStageCap : 1;
-    constexpr bool NT = STAGE == 2;
-    __shared__ uint32_t s_col[CAP];
+    constexpr bool NT = STAGE == 2 || STAGE == 4;
+    constexpr bool D16 = STAGE == 4; // 16-bit column deltas
+    __shared__ uint32_t s_col[D16 ? 1 : CAP];
+    __shared__ int16_t s_c16[D16 ? CAP : 1];
     __shared__ V s_val[CAP];
+    const int16_t *__restrict__ col16 = a.col16;
     for (int64_t base = r0; base < r1; base += RPB) {
         const int64_t i = base + lr;
         uint32_t jb = 0;
@@ -356,7 +363,10 @@ __global__ void __launch_bounds__(kBlock)
             __syncthreads(); // the previous tile's rows are done with the LDS copy
             if (staged)
                 for (uint32_t k = tid; k < jB - jA; k += kBlock) {
-                    s_col[k] = NT ? ld_nt(col + jA + k) : col[jA + k];
+                    if constexpr (D16)
+                        s_c16[k] = ld_nt(col16 + jA + k);
+                    else
+                        s_col[k] = NT ? ld_nt(col + jA + k) : col[jA + k];
                     s_val[k] = NT ? ld_nt(val + jA + k) : val[jA + k];
                 }
             __syncthreads();
@@ -366,10 +376,16 @@ __global__ void __launch_bounds__(kBlock)
             spmv_row<V, T, NV, VPT, EPI, ND, NT>(a, i, l0, j0, j1,
                                                  [&](uint32_t q, uint32_t &c, T &v) {
                                                      if (STAGE && staged) {
-                                                         c = s_col[q - jb];
+                                                         if constexpr (D16)
+                                                             c = (uint32_t)(i + s_c16[q - jb]);
+                                                         else
+                                                             c = s_col[q - jb];
                                                          v = to_t<T>(s_val[q - jb]);
                                                      } else {
-                                                         c = NT ? ld_nt(col + q) : col[q];
+                                                         if constexpr (D16)
+                                                             c = (uint32_t)(i + ld_nt(col16 + q));
+                                                         else
+                                                             c = NT ? ld_nt(col + q) : col[q];
                                                          v = to_t<T>(NT ? ld_nt(val + q) : val[q]);
                                                      }
                                                  },
@@ -388,9 +404,10 @@ __global__ void __launch_bounds__(kBlock)
 // memory (the x gathers) instead of three (row pointers, column indices, x).  Tile t+1
 // starts at tile t's last row pointer, so one scalar row-pointer load, issued a tile
 // ahead, bounds each copy.  A tile with more than CAP nonzeros reads them directly.
+// D16 (STAGE = 5): the column indices are copied as 16-bit offsets from the row.
 constexpr int kPipeNnzPerRow = 8;
 
-template <typename V, typename T, int NV, int VPT, int EPI, int ND>
+template <typename V, typename T, int NV, int VPT, int EPI, int ND, bool D16>
 __global__ void __launch_bounds__(kBlock)
     k_spmv_pipe(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
     static_assert(sizeof(V) % 4 == 0, "values are copied as dwords");
@@ -429,9 +446,18 @@ __global__ void __launch_bounds__(kBlock)
                 __builtin_amdgcn_global_load_lds(row + base + k0 + lane, &s_row[b][k0], 4, 0, 0);
         const uint32_t cnt = jB - jA;
         if (cnt <= (uint32_t)CAP) {
-            for (uint32_t k0 = wave * 64; k0 < cnt; k0 += kBlock)
-                if (k0 + lane < cnt)
-                    __builtin_amdgcn_global_load_lds(col + jA + k0 + lane, &s_col[b][k0], 4, 0, 0);
+            if constexpr (D16) {
+                // whole dwords of 16-bit offsets: [jA/2, (jB+1)/2) (the array is padded)
+                const uint32_t d0 = jA >> 1, nw = ((jB + 1) >> 1) - d0;
+                const uint32_t *cw = (const uint32_t *)a.col16 + d0;
+                for (uint32_t k0 = wave * 64; k0 < nw; k0 += kBlock)
+                    if (k0 + lane < nw)
+                        __builtin_amdgcn_global_load_lds(cw + k0 + lane, &s_col[b][k0], 4, 0, 0);
+            } else {
+                for (uint32_t k0 = wave * 64; k0 < cnt; k0 += kBlock)
+                    if (k0 + lane < cnt)
+                        __builtin_amdgcn_global_load_lds(col + jA + k0 + lane, &s_col[b][k0], 4, 0, 0);
+            }
             const uint32_t *vw = (const uint32_t *)val + (size_t)jA * VW;
             for (uint32_t k0 = wave * 64; k0 < cnt * VW; k0 += kBlock)
                 if (k0 + lane < cnt * VW)
@@ -467,13 +493,21 @@ __global__ void __launch_bounds__(kBlock)
             spmv_row<V, T, NV, VPT, EPI, ND, false>(a, i, l0, j0, j1,
                                                     [&](uint32_t q, uint32_t &c, T &v) {
                                                         if (staged) {
-                                                            c = s_col[b][q - jA];
+                                                            if constexpr (D16)
+                                                                c = (uint32_t)(
+                                                                    i + ((const int16_t *)s_col[b])
+                                                                            [q - 2 * (jA >> 1)]);
+                                                            else
+                                                                c = s_col[b][q - jA];
                                                             V w;
                                                             __builtin_memcpy(&w, &s_vw[b][(q - jA) * VW],
                                                                              sizeof(V));
                                                             v = to_t<T>(w);
                                                         } else {
-                                                            c = col[q];
+                                                            if constexpr (D16)
+                                                                c = (uint32_t)(i + a.col16[q]);
+                                                            else
+                                                                c = col[q];
                                                             v = to_t<T>(val[q]);
                                                         }
                                                     },
@@ -509,11 +543,12 @@ void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const Fin
     dim3 g(G), b(kBlock);
 #define XAMG_SPMV_CASE(E, D)                                                                       \
     if (epi == E && nd == D) {                                                                     \
-        if constexpr (STAGE == 3 && sizeof(V) % 4 == 0)                                            \
-            hipLaunchKernelGGL((k_spmv_pipe<V, T, NV, VPT, E, D>), g, b, 0, s, a, r, f, c.xcd_remap); \
+        if constexpr ((STAGE == 3 || STAGE == 5) && sizeof(V) % 4 == 0)                            \
+            hipLaunchKernelGGL((k_spmv_pipe<V, T, NV, VPT, E, D, STAGE == 5>), g, b, 0, s, a, r, f,    \
+                               c.xcd_remap);                                                        \
         else                                                                                       \
-            hipLaunchKernelGGL((k_spmv<V, T, NV, VPT, E, D, (STAGE == 3 ? 1 : STAGE)>), g, b, 0, s, a, \
-                               r, f, c.xcd_remap);                                                  \
+            hipLaunchKernelGGL((k_spmv<V, T, NV, VPT, E, D, (STAGE == 3 ? 1 : STAGE == 5 ? 4 : STAGE)>), \
+                               g, b, 0, s, a, r, f, c.xcd_remap);                                   \
         launch_finish<T, NV>(D, r, f, G, s);                                                       \
         return;                                                                                    \
     }
@@ -570,7 +605,8 @@ void spmv_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const FinA
 // Launch of the SpMV family for one staging mode (0: column indices and values read by
 // each row's lanes, 1: staged through LDS per tile of rows, 2: as 1 with non-temporal
 // streamed operands, 3: double-buffered LDS staging by direct global-to-LDS loads; BF16
-// values take mode 1 there).
+// values take mode 1 there; 4 and 5: modes 2 and 3 with 16-bit column offsets, BF16
+// values in mode 5 take mode 4).
 template <typename V, typename T, int STAGE>
 void launch_spmv_mode(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                       const FinArgs<T> &f, hipStream_t s) {
