@@ -56,6 +56,8 @@ void save(const std::string &key, const DevMatrix::Tune &t) {
     out << key << " " << t.vpt << " " << t.grid << " " << t.remap << " " << t.mode << "\n";
 }
 
+int default_vpt(int nv) { return nv == 1 ? 1 : nv == 2 ? 2 : 4; }
+
 std::vector<int> vpt_options(int nv) {
     switch (nv) {
     case 1:
@@ -112,6 +114,27 @@ struct TuneBufs {
 
 } // namespace
 
+// RHS per lane of the SpMV kernels with fused dots: the default width, narrowed while a
+// tile of rows (256 * VPT / nv of them) would hold more nonzeros on average than the
+// LDS staging capacity (long rows: 27-point stencils).
+int dot_vpt(const DevMatrix &A, int nv) {
+    const std::vector<int> opts = vpt_options(nv);
+    const double avg = A.n > 0 ? (double)A.nnz / (double)A.n : 0.0;
+    int v = default_vpt(nv);
+    for (;;) {
+        if (256.0 * v / nv * avg <= kSpmvStageCap)
+            break;
+        int prev = 0;
+        for (int o : opts)
+            if (o < v)
+                prev = o;
+        if (prev == 0)
+            break;
+        v = prev;
+    }
+    return v;
+}
+
 CacheFlush::CacheFlush() : src(kBytes), dst(kBytes) {} // contents irrelevant
 
 void CacheFlush::run() { launch_copy(src.p, dst.p, kBytes, Runtime::get().stream); }
@@ -160,10 +183,10 @@ DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
     DevMatrix::Tune best;
     float best_ms = 1e30f;
     // With fused dots the RHS-per-lane width and the workgroup count fix the order in
-    // which the dot products are summed, so they stay at the defaults there and the
-    // results do not depend on the tuning; the XCD mapping and the staging mode do not
-    // change the order.
-    const std::vector<int> vpts = nd > 0 ? std::vector<int>{0} : vpt_options(nv);
+    // which the dot products are summed, so they follow a rule of the matrix's shape
+    // (dot_vpt) and the default workgroup count, and the results do not depend on the
+    // tuning; the XCD mapping and the staging mode do not change the order.
+    const std::vector<int> vpts = nd > 0 ? std::vector<int>{dot_vpt(A, nv)} : vpt_options(nv);
     const std::vector<int> grids =
         nd > 0 ? std::vector<int>{launch_cfg().grid_red} : std::vector<int>{1024, 2048, 4096, 8192};
     std::vector<int> modes = {0, 1, 2, 3};
@@ -200,11 +223,13 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
     if (t.valid)
         return t;
     t.valid = true; // defaults unless tuned below
+    if (nd > 0) // the dot kernels' summation order does not depend on the tuning
+        t.vpt = dot_vpt(*this, nv);
     // small matrices: launch overheads dominate, keep defaults
     if (!autotune_enabled() || n < 65536)
         return t;
     std::ostringstream k;
-    k << "cold6_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
+    k << "cold7_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
       << nv << "_epi" << epi << "_nd" << nd;
     const std::string key = k.str();
     auto &c = cache();

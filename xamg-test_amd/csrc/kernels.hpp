@@ -162,6 +162,9 @@ struct LaunchCfg {
 
 LaunchCfg &launch_cfg();
 
+// Nonzeros of one row tile that SpMV staging modes 1, 2 and 4 hold in LDS.
+constexpr int kSpmvStageCap = 2048;
+
 // RHS per lane of the SpMV kernel for (nv, element size); 0 selects the default
 // (1 for nv=1, 2 for nv=2, 4 otherwise).  Compiled widths: nv=2 {1,2}, nv=4 {2,4},
 // nv=8 {2,4,8}, nv=16 {4,8,16}.

@@ -125,6 +125,10 @@ struct CacheFlush {
     void run();
 };
 
+// RHS per lane of the SpMV kernels with fused dots on this matrix (a rule of its shape,
+// so the dots' summation order never depends on timing).
+int dot_vpt(const DevMatrix &A, int nv);
+
 // Mean device time (ms) of one SpMV launch with the launch configuration in `a`: reps
 // back-to-back launches, or with `flush` each launch alone after a cache flush.
 template <typename T>

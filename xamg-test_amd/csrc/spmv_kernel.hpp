@@ -318,7 +318,7 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
     }
 }
 
-constexpr int kStageCap = 2048; // staged nonzeros per tile (16 per row at 128 rows)
+constexpr int kStageCap = kSpmvStageCap;
 
 template <typename V, typename T, int NV, int VPT, int EPI, int ND, int STAGE>
 __global__ void __launch_bounds__(kBlock)
@@ -382,11 +382,14 @@ __global__ void __launch_bounds__(kBlock)
                                                              c = s_col[q - jb];
                                                          v = to_t<T>(s_val[q - jb]);
                                                      } else {
+                                                         // a tile too large for LDS: plain loads
+                                                         // (non-temporal loads of entries a row
+                                                         // apart would fetch each line per lane)
                                                          if constexpr (D16)
-                                                             c = (uint32_t)(i + ld_nt(col16 + q));
+                                                             c = (uint32_t)(i + col16[q]);
                                                          else
-                                                             c = NT ? ld_nt(col + q) : col[q];
-                                                         v = to_t<T>(NT ? ld_nt(val + q) : val[q]);
+                                                             c = col[q];
+                                                         v = to_t<T>(val[q]);
                                                      }
                                                  },
                                                  dacc);
