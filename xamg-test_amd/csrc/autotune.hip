@@ -12,6 +12,7 @@
 // (XAMG_TUNE_FILE) across processes.
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -174,13 +175,19 @@ float time_spmv_cfg(DevMatrix &A, Prec vp, int nv, int epi, int nd, SpmvArgs<T> 
 
 namespace {
 
+struct TuneResult {
+    DevMatrix::Tune best;
+    std::vector<DevMatrix::Tune> finalists; // fastest of each (staging mode, XCD mapping)
+    std::vector<float> finalist_ms;
+};
+
 template <typename T>
-DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
+TuneResult run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
     KernelTimer::Pause quiet; // tuning launches are not part of any timed solve
     TuneBufs<T> bufs(A, nv);
     CacheFlush flush;
     SpmvArgs<T> a = bufs.args(epi);
-    DevMatrix::Tune best;
+    TuneResult res;
     float best_ms = 1e30f;
     // With fused dots the RHS-per-lane width and the workgroup count fix the order in
     // which the dot products are summed, so they follow a rule of the matrix's shape
@@ -195,30 +202,49 @@ DevMatrix::Tune run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
         modes.push_back(5);
     }
     for (int mode : modes)
-        for (int vpt : vpts)
-            for (int grid : grids)
-                for (int remap : {0, 1}) {
+        for (int remap : {0, 1}) {
+            DevMatrix::Tune fin;
+            float fin_ms = 1e30f;
+            for (int vpt : vpts)
+                for (int grid : grids) {
                     a.cfg_vpt = vpt;
                     a.cfg_grid = grid;
                     a.cfg_remap = remap;
                     a.cfg_mode = mode;
                     const float ms = time_spmv_cfg<T>(A, vp, nv, epi, nd, a, 4, &flush);
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best.vpt = vpt;
-                        best.grid = grid;
-                        best.remap = remap;
-                        best.mode = mode;
+                    if (ms < fin_ms) {
+                        fin_ms = ms;
+                        fin.vpt = vpt;
+                        fin.grid = grid;
+                        fin.remap = remap;
+                        fin.mode = mode;
+                        fin.valid = true;
                     }
                 }
-    best.valid = true;
-    return best;
+            res.finalists.push_back(fin);
+            res.finalist_ms.push_back(fin_ms);
+            if (fin_ms < best_ms) {
+                best_ms = fin_ms;
+                res.best = fin;
+            }
+        }
+    res.best.valid = true;
+    return res;
 }
 
 } // namespace
 
+bool online_tune_enabled() {
+    const char *e = getenv("XAMG_ONLINE_TUNE");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+static int tune_slot(Prec vp, int es, int nv, int epi, int nd) {
+    return (((int)vp * 2 + (es == 8 ? 0 : 1)) * (kMaxNV + 1) + nv) * 64 + epi * 4 + nd;
+}
+
 const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd) {
-    const int slot = (((int)vp * 2 + (es == 8 ? 0 : 1)) * (kMaxNV + 1) + nv) * 64 + epi * 4 + nd;
+    const int slot = tune_slot(vp, es, nv, epi, nd);
     Tune &t = tune[slot];
     if (t.valid)
         return t;
@@ -238,12 +264,116 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
         t = it->second;
         return t;
     }
-    Tune r = es == 8 ? run_tuning<double>(*this, vp, nv, epi, nd)
-                     : run_tuning<float>(*this, vp, nv, epi, nd);
-    tune[slot] = r;
-    c[key] = r;
-    save(key, r);
+    TuneResult r = es == 8 ? run_tuning<double>(*this, vp, nv, epi, nd)
+                           : run_tuning<float>(*this, vp, nv, epi, nd);
+    tune[slot] = r.best;
+    c[key] = r.best;
+    if (online_tune_enabled() && r.finalists.size() > 1) {
+        // the in-solve choice settles the class; the cold choice stands until then
+        Online &o = online[slot];
+        o.key = key;
+        o.cand = r.finalists;
+        o.cold_ms = r.finalist_ms;
+        o.got.assign(o.cand.size(), {});
+        o.issued.assign(o.cand.size(), 0);
+    } else {
+        save(key, r.best);
+    }
     return tune[slot];
+}
+
+// Folds the completed in-solve samples of a class into its record (a launch that did
+// almost nothing, one queued after the solver converged, is dropped and run again) and
+// settles the class once every finalist has its samples.
+static void online_harvest(DevMatrix::Online &o) {
+    size_t k = 0;
+    for (; k < o.pending.size(); k++) {
+        DevMatrix::Online::Sample &p = o.pending[k];
+        const hipError_t q = hipEventQuery(p.e1);
+        if (q == hipErrorNotReady)
+            break;
+        XAMG_HIP_CHECK(q);
+        float ms = 0;
+        XAMG_HIP_CHECK(hipEventElapsedTime(&ms, p.e0, p.e1));
+        if (ms < 0.25f * o.cold_ms[p.cand])
+            o.issued[p.cand]--;
+        else
+            o.got[p.cand].push_back(ms);
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+    o.pending.erase(o.pending.begin(), o.pending.begin() + k);
+}
+
+DevMatrix::Tune DevMatrix::launch_choice(Prec vp, int es, int nv, int epi, int nd,
+                                         bool allow_trial, int *trial, int *slot_out) {
+    *trial = -1;
+    const int slot = tune_slot(vp, es, nv, epi, nd);
+    *slot_out = slot;
+    const Tune t = tuned(vp, es, nv, epi, nd);
+    auto it = online.find(slot);
+    if (it == online.end() || it->second.done || !allow_trial)
+        return t;
+    Online &o = it->second;
+    online_harvest(o);
+    const int nc = (int)o.cand.size();
+    bool complete = true;
+    for (int c = 0; c < nc; c++)
+        complete = complete && (int)o.got[c].size() >= kOnlineSamples;
+    if (complete) {
+        int best = 0;
+        float best_med = 1e30f;
+        for (int c = 0; c < nc; c++) {
+            std::vector<float> v = o.got[c];
+            std::sort(v.begin(), v.end());
+            const float med = v[v.size() / 2];
+            if (med < best_med) {
+                best_med = med;
+                best = c;
+            }
+        }
+        o.done = true;
+        tune[slot] = o.cand[best];
+        cache()[o.key] = o.cand[best];
+        save(o.key, o.cand[best]);
+        return tune[slot];
+    }
+    for (int k = 0; k < nc; k++) {
+        const int c = (o.next + k) % nc;
+        if (o.issued[c] < kOnlineSamples) {
+            o.next = (c + 1) % nc;
+            o.issued[c]++;
+            *trial = c;
+            return o.cand[c];
+        }
+    }
+    return t; // every sample issued, some still running
+}
+
+void DevMatrix::online_begin(int slot, int trial) {
+    if (trial < 0)
+        return;
+    Online &o = online[slot];
+    Online::Sample p{trial, nullptr, nullptr};
+    XAMG_HIP_CHECK(hipEventCreate(&p.e0));
+    XAMG_HIP_CHECK(hipEventCreate(&p.e1));
+    XAMG_HIP_CHECK(hipEventRecord(p.e0, Runtime::get().stream));
+    o.pending.push_back(p);
+}
+
+void DevMatrix::online_end(int slot, int trial) {
+    if (trial < 0)
+        return;
+    Online &o = online[slot];
+    XAMG_HIP_CHECK(hipEventRecord(o.pending.back().e1, Runtime::get().stream));
+}
+
+DevMatrix::~DevMatrix() {
+    for (auto &kv : online)
+        for (Online::Sample &p : kv.second.pending) {
+            (void)hipEventDestroy(p.e0);
+            (void)hipEventDestroy(p.e1);
+        }
 }
 
 template <typename T>

@@ -312,6 +312,71 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
 // without per-workgroup L2 write-backs).  Wave w sums pairs p = w, w+16, ... over the
 // G workgroup partials in a fixed order, then one lane applies the scalar update.
 constexpr int kFinBlock = 1024;
+
+// LDS copies of the records the scalar update reads and writes (the Krylov scalars, the
+// convergence record, the FP32 scalars of continued refinement).  The update runs on one
+// lane as a chain of dependent accesses; on the records in HBM each would be a round
+// trip of about a microsecond (the previous finish kernel wrote them from another CU,
+// often another XCD).  The workgroup loads the records with one coalesced access per
+// lane, the lane works on the copies, and the workgroup writes them back the same way.
+// Pointers of the update's arguments that fall inside a record are redirected to its
+// copy.  Records that partially overlap are not copied (the update then works in HBM).
+struct FinShadow {
+    static constexpr int kRegions = 3;
+    static constexpr int kOff1 = sizeof(KrylovScalars<double>) / 4;
+    static constexpr int kOff2 = kOff1 + sizeof(ConvState) / 4;
+    static constexpr int kWords = kOff2 + sizeof(KrylovScalars<float>) / 4;
+    char *g[kRegions];
+    char *s[kRegions];
+    int words[kRegions];
+    bool on[kRegions];
+
+    __device__ void *map(void *p) const {
+        char *c = (char *)p;
+#pragma unroll
+        for (int k = 0; k < kRegions; k++)
+            if (on[k] && c >= g[k] && c < g[k] + 4 * words[k])
+                return s[k] + (c - g[k]);
+        return p;
+    }
+};
+static_assert(sizeof(KrylovScalars<double>) % 8 == 0 && sizeof(ConvState) % 8 == 0,
+              "records keep the 8-byte alignment of the next copy");
+static_assert(sizeof(KrylovScalars<double>) / 4 <= kFinBlock && sizeof(ConvState) / 4 <= kFinBlock,
+              "one record word per lane");
+
+// Fixed slots: the Krylov scalars, the convergence record, the FP32 scalars.  A record
+// at the address of an earlier one (the FP32 iteration's own scalars passed as both) is
+// copied once.
+template <typename T>
+__device__ inline bool fin_shadow_plan(const FinArgs<T> &f, uint32_t *lds, FinShadow &sh) {
+    sh.g[0] = (char *)f.ks;
+    sh.g[1] = (char *)f.cs;
+    sh.g[2] = (char *)f.aux;
+    sh.words[0] = sizeof(KrylovScalars<T>) / 4;
+    sh.words[1] = sizeof(ConvState) / 4;
+    sh.words[2] = sizeof(KrylovScalars<float>) / 4;
+    sh.s[0] = (char *)lds;
+    sh.s[1] = (char *)(lds + FinShadow::kOff1);
+    sh.s[2] = (char *)(lds + FinShadow::kOff2);
+#pragma unroll
+    for (int k = 0; k < FinShadow::kRegions; k++)
+        sh.on[k] = sh.g[k] != nullptr;
+#pragma unroll
+    for (int k = 1; k < FinShadow::kRegions; k++) {
+#pragma unroll
+        for (int j = 0; j < k; j++) {
+            if (!sh.on[j] || !sh.on[k])
+                continue;
+            if (sh.g[k] == sh.g[j] && sh.words[k] <= sh.words[j])
+                sh.on[k] = false;
+            else if (sh.g[k] < sh.g[j] + 4 * sh.words[j] && sh.g[j] < sh.g[k] + 4 * sh.words[k])
+                return false;
+        }
+    }
+    return true;
+}
+
 template <typename T, int NV, int ND>
 __global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, int G) {
     if (r.halt != nullptr && *r.halt)
@@ -321,7 +386,18 @@ __global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, i
     if (r.set_halt_if_done && r.cs != nullptr && r.cs->done)
         return;
     __shared__ double sums[ND * NV];
+    __shared__ uint64_t shadow_mem[(FinShadow::kWords + 1) / 2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    FinShadow sh;
+    const bool shadowed = fin_shadow_plan(f, (uint32_t *)shadow_mem, sh);
+    // record words of this lane, loaded now and stored to LDS after the partial sums
+    uint32_t rw[FinShadow::kRegions];
+    if (shadowed) {
+#pragma unroll
+        for (int k = 0; k < FinShadow::kRegions; k++)
+            if (sh.on[k] && tid < sh.words[k])
+                rw[k] = ((const uint32_t *)sh.g[k])[tid];
+    }
     for (int p = wave; p < ND * NV; p += kFinBlock / 64) {
         const double *src = r.ws.partials + (int64_t)p * r.ws.max_blocks;
         // fixed combination order: q[u] sums partials lane + 64 (u + 8k) in increasing k
@@ -356,11 +432,35 @@ __global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, i
         if (lane == 0)
             sums[p] = s;
     }
+    if (shadowed) {
+#pragma unroll
+        for (int k = 0; k < FinShadow::kRegions; k++)
+            if (sh.on[k] && tid < sh.words[k])
+                ((uint32_t *)sh.s[k])[tid] = rw[k];
+    }
     __syncthreads();
+    if (tid < ND * NV)
+        r.ws.result[tid] = sums[tid];
     if (tid == 0) {
-        for (int p = 0; p < ND * NV; p++)
-            r.ws.result[p] = sums[p];
-        apply_fin(f, sums, NV);
+        if (shadowed) {
+            FinArgs<T> g = f;
+            g.ks = (KrylovScalars<T> *)sh.map(f.ks);
+            g.cs = (ConvState *)sh.map(f.cs);
+            g.aux = sh.map(f.aux);
+            g.iconv = (T *)sh.map(f.iconv);
+            g.clear_on_fin = (int *)sh.map(f.clear_on_fin);
+            g.clear_on_fin2 = (int *)sh.map(f.clear_on_fin2);
+            apply_fin(g, sums, NV);
+        } else {
+            apply_fin(f, sums, NV);
+        }
+    }
+    if (shadowed) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < FinShadow::kRegions; k++)
+            if (sh.on[k] && tid < sh.words[k])
+                ((uint32_t *)sh.g[k])[tid] = ((const uint32_t *)sh.s[k])[tid];
     }
 }
 

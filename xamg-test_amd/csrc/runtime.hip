@@ -362,8 +362,28 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
     a.col = A.col.as<uint32_t>();
     a.col16 = A.col16.as<int16_t>();
     a.n = A.n;
+    // matrix values of the requested precision (built on first use, before any timing)
+    if constexpr (std::is_same<T, double>::value) {
+        XAMG_REQUIRE(vp == Prec::F64, "FP64 vectors need FP64 matrix values");
+        a.val = A.values(Prec::F64);
+    } else if (vp == Prec::BF16) {
+        a.val = A.values(Prec::BF16);
+    } else if (vp == Prec::F32S) {
+        a.val = A.values(Prec::F32S);
+        a.prescaled = 1;
+    } else {
+        a.val = A.values(Prec::F32);
+    }
+    int trial = -1, slot = -1;
     if (a.cfg_vpt == 0 && a.cfg_grid == 0) {
-        const DevMatrix::Tune &t = A.tuned(vp, (int)sizeof(T), nv, epi, nd);
+        // in-solve trials only where a launch always runs (no gate) and its events can be
+        // recorded (not while the stream is captured into a graph)
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        XAMG_HIP_CHECK(hipStreamIsCapturing(s, &cap));
+        const bool allow = r.gate == nullptr && launch_cfg().spmv_stage < 0 &&
+                           cap == hipStreamCaptureStatusNone;
+        const DevMatrix::Tune t =
+            A.launch_choice(vp, (int)sizeof(T), nv, epi, nd, allow, &trial, &slot);
         a.cfg_vpt = t.vpt;
         a.cfg_grid = t.grid;
         a.cfg_remap = t.remap;
@@ -384,23 +404,15 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
                  (long long)A.n);
         tok = kt.begin(cls, spmv_launch_bytes<T>(A, vp, epi, nd, nv, a));
     }
+    A.online_begin(slot, trial);
     if constexpr (std::is_same<T, double>::value) {
-        XAMG_REQUIRE(vp == Prec::F64, "FP64 vectors need FP64 matrix values");
-        a.val = A.values(Prec::F64);
         launch_spmv<double, double>(epi, nd, nv, a, r, f, s);
+    } else if (vp == Prec::BF16) {
+        launch_spmv<__hip_bfloat16, float>(epi, nd, nv, a, r, f, s);
     } else {
-        if (vp == Prec::BF16) {
-            a.val = A.values(Prec::BF16);
-            launch_spmv<__hip_bfloat16, float>(epi, nd, nv, a, r, f, s);
-        } else if (vp == Prec::F32S) {
-            a.val = A.values(Prec::F32S);
-            a.prescaled = 1;
-            launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
-        } else {
-            a.val = A.values(Prec::F32);
-            launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
-        }
+        launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
     }
+    A.online_end(slot, trial);
     kt.end(tok);
 }
 

@@ -111,11 +111,47 @@ struct DevMatrix {
     };
     std::map<int, Tune> tune;
     const Tune &tuned(Prec vp, int es, int nv, int epi, int nd);
+
+    // In-solve choice among the finalists of the cold timing (the fastest configuration
+    // of each staging mode and XCD mapping): the first launches of a class inside the
+    // solver rotate over the finalists with HIP events around each launch, and after
+    // kOnlineSamples timed launches of every finalist the one with the smallest median
+    // device time becomes the class's configuration.  A single launch timed after a
+    // cache flush misranks the modes: inside a solver the previous kernel leaves part of
+    // this launch's operands in the last-level cache.  Every finalist computes the same
+    // bits (row sums in column order; the dot kernels' finalists share the workgroup
+    // count and RHS width that fix the summation order).
+    struct Online {
+        struct Sample {
+            int cand;
+            hipEvent_t e0, e1;
+        };
+        std::string key;
+        std::vector<Tune> cand;
+        std::vector<float> cold_ms;           // cold time of each finalist
+        std::vector<std::vector<float>> got;  // in-solve samples per finalist
+        std::vector<int> issued;
+        std::vector<Sample> pending;
+        int next = 0;
+        bool done = false;
+    };
+    std::map<int, Online> online;
+    static constexpr int kOnlineSamples = 3;
+    // configuration of the next launch of a class (a finalist while the in-solve choice
+    // runs); *trial >= 0 (the finalist's index) asks for online_begin() right before and
+    // online_end() right after the launch
+    Tune launch_choice(Prec vp, int es, int nv, int epi, int nd, bool allow_trial, int *trial,
+                       int *slot);
+    void online_begin(int slot, int trial);
+    void online_end(int slot, int trial);
+    ~DevMatrix();
 };
 
-// Autotuner switches: XAMG_AUTOTUNE=0 disables it; XAMG_TUNE_FILE names a text file
-// that caches results across processes.
+// Autotuner switches: XAMG_AUTOTUNE=0 disables it; XAMG_ONLINE_TUNE=0 keeps the cold
+// timing's choice (no in-solve refinement); XAMG_TUNE_FILE names a text file that
+// caches results across processes.
 bool autotune_enabled();
+bool online_tune_enabled();
 
 // A 512 MiB device-to-device copy that evicts the L2 caches and the last-level cache.
 struct CacheFlush {
