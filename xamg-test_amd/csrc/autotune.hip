@@ -268,16 +268,16 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
                            : run_tuning<float>(*this, vp, nv, epi, nd);
     tune[slot] = r.best;
     c[key] = r.best;
+    // the tuning file gets the cold choice now and the in-solve choice, which replaces it
+    // when the file is read back, once settled
+    save(key, r.best);
     if (online_tune_enabled() && r.finalists.size() > 1) {
-        // the in-solve choice settles the class; the cold choice stands until then
         Online &o = online[slot];
         o.key = key;
         o.cand = r.finalists;
         o.cold_ms = r.finalist_ms;
         o.got.assign(o.cand.size(), {});
         o.issued.assign(o.cand.size(), 0);
-    } else {
-        save(key, r.best);
     }
     return tune[slot];
 }

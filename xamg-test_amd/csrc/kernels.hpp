@@ -97,6 +97,23 @@ struct RedArgs {
     ConvState *cs = nullptr;
 };
 
+// The flags a kernel tests before any work: the loop's halt flag, the gate of a
+// conditional step, the solver's done flag.  All three are loaded before the first test,
+// so they cost one round trip to memory together (the previous kernel wrote them, often
+// from another XCD) instead of one each; an absent flag reads a constant.
+struct StartFlags {
+    int halt, gate, done;
+};
+static __device__ const int kStartFlagConst[2] = {0, 1};
+
+__device__ __forceinline__ StartFlags start_flags(const RedArgs &r) {
+    const int *hp = r.halt != nullptr ? r.halt : &kStartFlagConst[0];
+    const int *gp = r.gate != nullptr ? r.gate : &kStartFlagConst[1];
+    const int *dp =
+        (r.set_halt_if_done && r.cs != nullptr) ? &r.cs->done : &kStartFlagConst[0];
+    return StartFlags{*hp, *gp, *dp};
+}
+
 template <typename T>
 struct SpmvArgs {
     const uint32_t *row;

@@ -323,9 +323,8 @@ constexpr int kStageCap = kSpmvStageCap;
 template <typename V, typename T, int NV, int VPT, int EPI, int ND, int STAGE>
 __global__ void __launch_bounds__(kBlock)
     k_spmv(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
-    if (r.halt != nullptr && *r.halt)
-        return;
-    if (r.gate != nullptr && !*r.gate)
+    const StartFlags fl = start_flags(r);
+    if (fl.halt || !fl.gate)
         return;
     constexpr int LR = NV / VPT;
     constexpr int RPB = kBlock / LR;
@@ -414,9 +413,8 @@ template <typename V, typename T, int NV, int VPT, int EPI, int ND, bool D16>
 __global__ void __launch_bounds__(kBlock)
     k_spmv_pipe(SpmvArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
     static_assert(sizeof(V) % 4 == 0, "values are copied as dwords");
-    if (r.halt != nullptr && *r.halt)
-        return;
-    if (r.gate != nullptr && !*r.gate)
+    const StartFlags fl = start_flags(r);
+    if (fl.halt || !fl.gate)
         return;
     constexpr int LR = NV / VPT;
     constexpr int RPB = kBlock / LR;
@@ -438,8 +436,10 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int v = 0; v < VPT; v++)
             dacc[k][v] = 0.0;
+    // 16-bit offsets: a staged tile's [jA/2, (jB+1)/2) holds at most CAP/2 + 1 dwords
+    constexpr int CAPC = D16 ? CAP / 2 + 2 : CAP;
     __shared__ uint32_t s_row[2][RPB + 1];
-    __shared__ uint32_t s_col[2][CAP];
+    __shared__ uint32_t s_col[2][CAPC];
     __shared__ __align__(16) uint32_t s_vw[2][CAP * VW];
     // copy of tile rows [base, bend] (row pointers) and its nonzeros [jA, jB) into buffer b
     auto issue = [&](int b, int64_t base, int64_t bend, uint32_t jA, uint32_t jB) {
