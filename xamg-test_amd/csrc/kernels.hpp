@@ -24,8 +24,20 @@ enum Epi : int {
     EPI_JACOBI = 3,   // y = xin + (w*d_i)*(b - acc)                  (blas_merged_jacobi.inl:80-85)
     EPI_CHEB = 4,     // y = c*(b*d_i) + 1*(acc*d_i), x pre-scaled by d_col (chebyshev.inl:172-178)
     EPI_CHEB_LAST = 5, // as EPI_CHEB then y = 1*(y*d_i) [+ 1*xold]  (chebyshev.inl:180-181)
-    EPI_RESID_MX = 6   // FP64 r = b - acc stored as FP32 (yo, yo2, yo3); dot in FP64
+    EPI_RESID_MX = 6,  // FP64 r = b - acc stored as FP32 (yo, yo2, yo3); dot in FP64
+    // Horner steps after the first on the scaled operator D^{-1/2} A D^{-1/2}: x is used
+    // as gathered (no column scaling), so the kernels carry no scaling code and need far
+    // fewer registers (more waves in flight)
+    EPI_CHEB_S = 7,     // y = c*(b*d_i) + 1*acc
+    EPI_CHEB_LAST_S = 8 // as EPI_CHEB_S then y = 1*(y*d_i) [+ 1*xold]
 };
+
+// Epilogue families of the Chebyshev steps.
+constexpr bool epi_cheb(int e) {
+    return e == EPI_CHEB || e == EPI_CHEB_LAST || e == EPI_CHEB_S || e == EPI_CHEB_LAST_S;
+}
+constexpr bool epi_cheb_last(int e) { return e == EPI_CHEB_LAST || e == EPI_CHEB_LAST_S; }
+constexpr bool epi_cheb_scaled(int e) { return e == EPI_CHEB_S || e == EPI_CHEB_LAST_S; }
 
 // Elementwise ops on [n][nv] vectors with per-RHS coefficients (nullptr = 1).
 enum Ew : int {

@@ -308,8 +308,9 @@ static const char *prec_tag(Prec p) {
 }
 
 static const char *epi_tag(int epi) {
-    static const char *t[] = {"axy", "axpy", "resid", "jacobi", "cheb", "cheb_last", "resid_mx"};
-    return epi >= 0 && epi < 7 ? t[epi] : "?";
+    static const char *t[] = {"axy",      "axpy",   "resid",      "jacobi",   "cheb",
+                              "cheb_last", "resid_mx", "cheb_s", "cheb_last_s"};
+    return epi >= 0 && epi < 9 ? t[epi] : "?";
 }
 
 // Algorithmic HBM bytes of one SpMV launch: row pointers, column indices and values
@@ -331,11 +332,11 @@ static double spmv_launch_bytes(const DevMatrix &A, Prec vp, int epi, int nd, in
         b += E; // b (the first Chebyshev step gathers b itself as x)
     if (epi == EPI_JACOBI)
         b += E + Dv; // previous iterate, inverse diagonal
-    if (epi == EPI_CHEB || epi == EPI_CHEB_LAST) {
+    if (epi_cheb(epi)) {
         b += Dv;
-        if (a.dcol != nullptr && a.dcol != a.d)
+        if (!epi_cheb_scaled(epi) && a.dcol != nullptr && a.dcol != a.d)
             b += Dv; // column scaling held apart from the row scaling
-        if (epi == EPI_CHEB_LAST && a.xin != nullptr)
+        if (epi_cheb_last(epi) && a.xin != nullptr)
             b += E;
     }
     if (nd > 0 && a.w != nullptr)

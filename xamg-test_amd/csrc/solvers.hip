@@ -472,14 +472,18 @@ class Chebyshev : public Solver<T> {
                 s.d = d;
                 s.dcol = d;
                 s.coef = c[q];
+                // steps after the first on the scaled operator gather x as it is
+                const bool lean = !first && hprec == Prec::F32S;
                 if (q > 0) {
                     s.y = t2;
-                    spmv<T>(*A, hprec, EPI_CHEB, 0, nv, s, red_args(H), FinArgs<T>{});
+                    spmv<T>(*A, hprec, lean ? EPI_CHEB_S : EPI_CHEB, 0, nv, s, red_args(H),
+                            FinArgs<T>{});
                     std::swap(t1, t2);
                 } else {
                     s.y = x;
                     s.xin = xzero ? nullptr : x;
-                    spmv<T>(*A, hprec, EPI_CHEB_LAST, 0, nv, s, red_args(H), FinArgs<T>{});
+                    spmv<T>(*A, hprec, lean ? EPI_CHEB_LAST_S : EPI_CHEB_LAST, 0, nv, s,
+                            red_args(H), FinArgs<T>{});
                 }
             }
         }

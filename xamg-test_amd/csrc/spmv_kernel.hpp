@@ -177,7 +177,7 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
     // the epilogue's own row operands are loaded before the gathers so their
     // latency overlaps the nonzero loop
     constexpr bool kNeedB = EPI != EPI_AXY && EPI != EPI_AXPY;
-    constexpr bool kNeedD = EPI == EPI_JACOBI || EPI == EPI_CHEB || EPI == EPI_CHEB_LAST;
+    constexpr bool kNeedD = EPI == EPI_JACOBI || epi_cheb(EPI);
     T pb[VPT], pw[VPT], px[VPT];
     T pdi = (T)0;
     if constexpr (kNeedB)
@@ -186,7 +186,7 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
         pdi = a.d[i];
     if constexpr (EPI == EPI_JACOBI) {
         ld_vec<T, VPT, NT>(a.xin + e, px);
-    } else if constexpr (EPI == EPI_CHEB_LAST) {
+    } else if constexpr (epi_cheb_last(EPI)) {
         if (a.xin != nullptr)
             ld_vec<T, VPT, NT>(a.xin + e, px);
     }
@@ -266,10 +266,12 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
 #pragma unroll
         for (int v = 0; v < VPT; v++) {
             const T rr = pb[v] * di;
-            const T t2 = a.prescaled ? acc[v] : acc[v] * di;
+            T t2 = acc[v];
+            if constexpr (!epi_cheb_scaled(EPI))
+                t2 = a.prescaled ? acc[v] : acc[v] * di;
             out[v] = a.coef * rr + (T)1.0 * t2;
         }
-        if constexpr (EPI == EPI_CHEB_LAST) {
+        if constexpr (epi_cheb_last(EPI)) {
 #pragma unroll
             for (int v = 0; v < VPT; v++)
                 out[v] = out[v] * di;
@@ -564,6 +566,8 @@ void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const Fin
     XAMG_SPMV_CASE(EPI_JACOBI, 0)
     XAMG_SPMV_CASE(EPI_CHEB, 0)
     XAMG_SPMV_CASE(EPI_CHEB_LAST, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_S, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_LAST_S, 0)
     if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value) {
         XAMG_SPMV_CASE(EPI_RESID_MX, 0)
         XAMG_SPMV_CASE(EPI_RESID_MX, 1)
