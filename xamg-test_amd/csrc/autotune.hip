@@ -350,6 +350,11 @@ DevMatrix::Tune DevMatrix::launch_choice(Prec vp, int es, int nv, int epi, int n
     return t; // every sample issued, some still running
 }
 
+bool DevMatrix::choice_settled(Prec vp, int es, int nv, int epi, int nd) {
+    auto it = online.find(tune_slot(vp, es, nv, epi, nd));
+    return it == online.end() || it->second.done;
+}
+
 void DevMatrix::online_begin(int slot, int trial) {
     if (trial < 0)
         return;
@@ -387,6 +392,13 @@ static float bench_epi(DevMatrix &A, Prec vp, int nv, int epi, int nd, int cold,
         a.cfg_grid = cfg[1];
         a.cfg_remap = cfg[2];
         a.cfg_mode = cfg[3];
+    } else {
+        // the class's current configuration, without in-solve trial launches
+        const DevMatrix::Tune t = A.tuned(vp, (int)sizeof(T), nv, epi, nd);
+        a.cfg_vpt = t.vpt > 0 ? t.vpt : spmv_vpt_choice(nv, (int)sizeof(T));
+        a.cfg_grid = t.grid > 0 ? t.grid : (nd > 0 ? launch_cfg().grid_red : launch_cfg().grid_spmv);
+        a.cfg_remap = t.remap;
+        a.cfg_mode = t.mode;
     }
     std::unique_ptr<CacheFlush> flush;
     if (cold)

@@ -816,8 +816,15 @@ int xamgx_bench_spmv(XAMG_matrix_id mid, int prec, int nv, int iters, double *av
             fill<double>(dx.as<double>(), A->n, nv, 1.0);
         else
             fill<float>(dx.as<float>(), A->n, nv, 1.0f);
-        for (int i = 0; i < 10; i++)
+        // warm-up, until the in-solve choice of the launch configuration has settled (its
+        // trial launches would otherwise fall into the timed loop)
+        const Prec vp = prec == 0 ? Prec::F64 : prec == 1 ? Prec::F32 : Prec::BF16;
+        for (int i = 0; i < 10 || (!A->choice_settled(vp, (int)es, nv, EPI_AXY, 0) && i < 1000);
+             i++) {
             launch();
+            if (i % 16 == 15)
+                rt.sync();
+        }
         hipEvent_t e0, e1;
         XAMG_HIP_CHECK(hipEventCreate(&e0));
         XAMG_HIP_CHECK(hipEventCreate(&e1));

@@ -143,6 +143,8 @@ struct DevMatrix {
     Tune launch_choice(Prec vp, int es, int nv, int epi, int nd, bool allow_trial, int *trial,
                        int *slot);
     void online_begin(int slot, int trial);
+    // false while the in-solve choice of a class is still trying its finalists
+    bool choice_settled(Prec vp, int es, int nv, int epi, int nd);
     void online_end(int slot, int trial);
     ~DevMatrix();
 };
