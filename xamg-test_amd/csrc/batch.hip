@@ -121,11 +121,11 @@ __device__ __forceinline__ void acc_slot(double (&acc)[kBatchMaxNV], int l, doub
 // y = epilogue(A x) over the tiles of the running systems.  One lane per vector element
 // (row, RHS): lanes of one row read the same column / value entries; the row's nonzeros
 // are loaded U at a time and summed in ascending column order (csr_matvec.inl:213-231,
-// x * val).
-template <typename V, typename T, int EPI, int ND>
+// x * val).  U is the smallest compiled chunk that holds the batch's longest row (5 for
+// 5-point stencils), so a row costs one round of loads and no padding loads.
+template <typename V, typename T, int EPI, int ND, int U>
 __global__ void __launch_bounds__(BT)
     kb_spmv(BLayout L, const int *halt, BSpmvArgs<T> a, double *part) {
-    constexpr int U = 8;
     const V *val = (const V *)a.val;
     for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
         const int s = L.tsys[t];
@@ -610,9 +610,12 @@ void Batch::build(int ns, const int *sys_rows, const int *nvs, const int *r, con
     hrow.resize(nrows + 1);
     hcol.resize(nnz);
     hval.assign(v, v + nnz);
+    max_row_nnz = 0;
     for (int64_t i = 0; i <= nrows; i++) {
         XAMG_REQUIRE(i == 0 || r[i] >= r[i - 1], "batch: row pointers must be non-decreasing");
         hrow[i] = (uint32_t)r[i];
+        if (i > 0)
+            max_row_nnz = std::max(max_row_nnz, r[i] - r[i - 1]);
     }
     for (int s = 0; s < ns; s++)
         for (int64_t i = sys_rows[s]; i < sys_rows[s + 1]; i++)
@@ -678,6 +681,7 @@ BLayout Batch::layout() const {
     L.stile = stile.as<int>();
     L.row = row.as<uint32_t>();
     L.col = col.as<uint32_t>();
+    L.max_row_nnz = max_row_nnz;
     return L;
 }
 
@@ -721,7 +725,14 @@ template <typename V, typename T, int EPI, int ND>
 void launch_bspmv(const BLayout &L, const int *halt, const BSpmvArgs<T> &a, double *part,
                   hipStream_t st) {
     const int G = std::max(1, std::min(batch_cfg().grid, L.ntiles));
-    hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+    if (L.max_row_nnz <= 3)
+        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 3>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+    else if (L.max_row_nnz <= 5)
+        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 5>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+    else if (L.max_row_nnz <= 7)
+        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 7>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+    else
+        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 9>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
 }
 template <typename T, int OP, int ND>
 void launch_bew(const BLayout &L, const int *halt, const BEwArgs<T> &a, double *part,

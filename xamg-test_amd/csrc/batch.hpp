@@ -65,6 +65,7 @@ struct BLayout {
     const uint32_t *trow = nullptr; // tile -> first global row [ntiles+1] (tiles are contiguous)
     const int *stile = nullptr;     // system -> first tile [nsys+1]
     const uint32_t *row = nullptr, *col = nullptr;
+    int max_row_nnz = 0; // longest row of the batch (fixes the SpMV's nonzero chunk)
 };
 
 // Device view of the per-system solver state.
@@ -90,6 +91,7 @@ class Batch {
     std::vector<int> h_nv;
     std::vector<int64_t> h_voff;
     std::vector<double> hval;
+    int max_row_nnz = 0;
     DBuf srow, nv, voff, row, col, val64, val32, dinv64, dinv32;
     // tiling (rebuilt when the tile size changes)
     int tile_rows = 0, ntiles = 0;
