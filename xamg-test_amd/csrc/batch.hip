@@ -150,6 +150,19 @@ __global__ void __launch_bounds__(BT)
             const int64_t e = vb + (int64_t)rs * nv + q;
             const T *xb = a.x + vb + l;
             T acc = (T)0;
+            // the epilogue's and the dots' row operands are loaded before the gathers, so
+            // their latency overlaps the nonzero loop
+            T pb = (T)0, px = (T)0, pd = (T)0, pw = (T)0;
+            if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX || EPI == EPI_JACOBI)
+                pb = a.b[e];
+            if constexpr (EPI == EPI_JACOBI) {
+                px = a.xin[e];
+                pd = a.d[i];
+            }
+            if constexpr (ND == 3)
+                pw = a.w[e];
+            else if constexpr (ND == 1)
+                pw = a.w != nullptr ? a.w[e] : (T)0;
             const uint32_t j0 = L.row[i], j1 = L.row[i + 1];
             for (uint32_t j = j0; j < j1; j += U) {
                 uint32_t c[U];
@@ -172,9 +185,9 @@ __global__ void __launch_bounds__(BT)
             if constexpr (EPI == EPI_AXY) {
                 out = acc;
             } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
-                out = (T)1.0 * a.b[e] + (T)(-1.0) * acc;
+                out = (T)1.0 * pb + (T)(-1.0) * acc;
             } else { // EPI_JACOBI: x_new = x + (w d)(b - A x)
-                out = a.xin[e] + (a.wcoef[s * kBatchMaxNV + l] * a.d[i]) * (a.b[e] - acc);
+                out = px + (a.wcoef[s * kBatchMaxNV + l] * pd) * (pb - acc);
             }
             if constexpr (EPI == EPI_RESID_MX) {
                 const float o = (float)out;
@@ -193,10 +206,10 @@ __global__ void __launch_bounds__(BT)
                     a.y3[e] = out;
             }
             if constexpr (ND == 1) {
-                const double wv = a.w != nullptr ? (double)a.w[e] : (double)out;
+                const double wv = a.w != nullptr ? (double)pw : (double)out;
                 acc_slot(dacc[0], l, (double)out * wv);
             } else if constexpr (ND == 3) {
-                const double o = (double)out, wv = (double)a.w[e];
+                const double o = (double)out, wv = (double)pw;
                 acc_slot(dacc[0], l, o * wv);
                 acc_slot(dacc[1], l, o * o);
                 acc_slot(dacc[2], l, wv * wv);
@@ -251,44 +264,70 @@ __global__ void __launch_bounds__(BT)
 #pragma unroll
             for (int l = 0; l < kBatchMaxNV; l++)
                 dacc[k][l] = 0.0;
+        // Every operand of an element is loaded before the first test or use of any of
+        // them (absent optional operands read x again), so an element costs one round
+        // trip to memory; a load behind a branch would cost a second one.
+        const T *zsrc = a.keep != nullptr ? a.z : a.x;
+        const T *wsrc = (ND > 0 && a.w != nullptr) ? a.w : a.x;
         for (uint32_t q = q0 + threadIdx.x; q < q1; q += BT) {
             const uint32_t li = div_nv(q, nv);
             const int l = (int)(q - li * nv);
             const int64_t e = eb + q;
+            T xe = (T)0, ye = (T)0, ze = (T)0, we = (T)0, de = (T)0;
+            float xo = 0.0f, yo = 0.0f;
+            if constexpr (OP == EW_AXPBY_Z) {
+                xe = a.x[e];
+                ye = a.y[e];
+                ze = zsrc[e];
+            } else if constexpr (OP == EW_AXPBYPCZ) {
+                xe = a.x[e];
+                ye = a.y[e];
+                ze = a.z[e];
+            } else if constexpr (OP == EW_JACOBI0) {
+                xe = a.x[e];
+                de = a.d[rs + li];
+            } else if constexpr (OP == EW_COPY3) {
+                xe = a.x[e];
+            } else if constexpr (OP == EW_MX_XUPD) {
+                xo = ((const float *)a.xo)[e];
+                yo = ((const float *)a.yo)[e];
+                ze = a.z[e];
+            }
+            if constexpr (ND > 0)
+                we = wsrc[e];
             T out;
             if constexpr (OP == EW_FILL) {
                 out = a.c0;
             } else if constexpr (OP == EW_AXPBY_Z) {
-                out = keep ? a.z[e] : sel4(ca, l) * a.x[e] + sel4(cb, l) * a.y[e];
+                out = keep ? ze : sel4(ca, l) * xe + sel4(cb, l) * ye;
                 if (copy && a.y2 != nullptr)
                     a.y2[e] = out;
             } else if constexpr (OP == EW_AXPBYPCZ) {
                 if (copy)
-                    out = a.x[e];
+                    out = xe;
                 else
                     out = (keep || sel4(frozen, l))
-                              ? a.z[e]
-                              : sel4(ca, l) * a.x[e] + sel4(cb, l) * a.y[e] + sel4(cc, l) * a.z[e];
+                              ? ze
+                              : sel4(ca, l) * xe + sel4(cb, l) * ye + sel4(cc, l) * ze;
             } else if constexpr (OP == EW_JACOBI0) {
-                out = (sel4(ca, l) * a.d[rs + li]) * a.x[e];
+                out = (sel4(ca, l) * de) * xe;
             } else if constexpr (OP == EW_COPY3) {
-                out = (T)1.0 * a.x[e];
+                out = (T)1.0 * xe;
                 if (a.y2 != nullptr)
                     a.y2[e] = out;
                 if (a.y3 != nullptr)
                     a.y3[e] = out;
             } else { // EW_MX_XUPD
-                out = sel4(frozen, l) ? a.z[e]
-                                      : (T)sel4(ao, l) * (T)((const float *)a.xo)[e] +
-                                            (T)sel4(bo, l) * (T)((const float *)a.yo)[e] +
-                                            (T)1.0 * a.z[e];
+                out = sel4(frozen, l) ? ze
+                                      : (T)sel4(ao, l) * (T)xo + (T)sel4(bo, l) * (T)yo +
+                                            (T)1.0 * ze;
             }
             a.z[e] = out;
             if constexpr (ND == 1) {
-                const double wv = (a.w != nullptr && !copy) ? (double)a.w[e] : (double)out;
+                const double wv = (a.w != nullptr && !copy) ? (double)we : (double)out;
                 acc_slot(dacc[0], l, (double)out * wv);
             } else if constexpr (ND == 3) {
-                const double o = (double)out, wv = copy ? o : (double)a.w[e];
+                const double o = (double)out, wv = copy ? o : (double)we;
                 acc_slot(dacc[0], l, o * wv);
                 acc_slot(dacc[1], l, o * o);
                 acc_slot(dacc[2], l, wv * wv);
