@@ -265,72 +265,91 @@ __global__ void __launch_bounds__(BT)
             for (int l = 0; l < kBatchMaxNV; l++)
                 dacc[k][l] = 0.0;
         // Every operand of an element is loaded before the first test or use of any of
-        // them (absent optional operands read x again), so an element costs one round
-        // trip to memory; a load behind a branch would cost a second one.
+        // them (absent optional operands read x again), and the K elements of a thread in
+        // a step (BT apart) are all loaded before the first store: the elements of a tile
+        // cost one round trip to memory per step instead of one each.  The dots add the
+        // elements in the thread's ascending order.
+        constexpr int K = 4;
         const T *zsrc = a.keep != nullptr ? a.z : a.x;
         const T *wsrc = (ND > 0 && a.w != nullptr) ? a.w : a.x;
-        for (uint32_t q = q0 + threadIdx.x; q < q1; q += BT) {
-            const uint32_t li = div_nv(q, nv);
-            const int l = (int)(q - li * nv);
-            const int64_t e = eb + q;
-            T xe = (T)0, ye = (T)0, ze = (T)0, we = (T)0, de = (T)0;
-            float xo = 0.0f, yo = 0.0f;
-            if constexpr (OP == EW_AXPBY_Z) {
-                xe = a.x[e];
-                ye = a.y[e];
-                ze = zsrc[e];
-            } else if constexpr (OP == EW_AXPBYPCZ) {
-                xe = a.x[e];
-                ye = a.y[e];
-                ze = a.z[e];
-            } else if constexpr (OP == EW_JACOBI0) {
-                xe = a.x[e];
-                de = a.d[rs + li];
-            } else if constexpr (OP == EW_COPY3) {
-                xe = a.x[e];
-            } else if constexpr (OP == EW_MX_XUPD) {
-                xo = ((const float *)a.xo)[e];
-                yo = ((const float *)a.yo)[e];
-                ze = a.z[e];
+        for (uint32_t qb = q0 + threadIdx.x; qb < q1; qb += K * BT) {
+            T xe[K], ye[K], ze[K], we[K], de[K];
+            float xo[K], yo[K];
+#pragma unroll
+            for (int u = 0; u < K; u++) {
+                // an element past the tile reads the step's first element again
+                const uint32_t q = qb + u * BT < q1 ? qb + u * BT : qb;
+                const uint32_t li = div_nv(q, nv);
+                const int64_t e = eb + q;
+                xe[u] = ye[u] = ze[u] = we[u] = de[u] = (T)0;
+                xo[u] = yo[u] = 0.0f;
+                if constexpr (OP == EW_AXPBY_Z) {
+                    xe[u] = a.x[e];
+                    ye[u] = a.y[e];
+                    ze[u] = zsrc[e];
+                } else if constexpr (OP == EW_AXPBYPCZ) {
+                    xe[u] = a.x[e];
+                    ye[u] = a.y[e];
+                    ze[u] = a.z[e];
+                } else if constexpr (OP == EW_JACOBI0) {
+                    xe[u] = a.x[e];
+                    de[u] = a.d[rs + li];
+                } else if constexpr (OP == EW_COPY3) {
+                    xe[u] = a.x[e];
+                } else if constexpr (OP == EW_MX_XUPD) {
+                    xo[u] = ((const float *)a.xo)[e];
+                    yo[u] = ((const float *)a.yo)[e];
+                    ze[u] = a.z[e];
+                }
+                if constexpr (ND > 0)
+                    we[u] = wsrc[e];
             }
-            if constexpr (ND > 0)
-                we = wsrc[e];
-            T out;
-            if constexpr (OP == EW_FILL) {
-                out = a.c0;
-            } else if constexpr (OP == EW_AXPBY_Z) {
-                out = keep ? ze : sel4(ca, l) * xe + sel4(cb, l) * ye;
-                if (copy && a.y2 != nullptr)
-                    a.y2[e] = out;
-            } else if constexpr (OP == EW_AXPBYPCZ) {
-                if (copy)
-                    out = xe;
-                else
-                    out = (keep || sel4(frozen, l))
-                              ? ze
-                              : sel4(ca, l) * xe + sel4(cb, l) * ye + sel4(cc, l) * ze;
-            } else if constexpr (OP == EW_JACOBI0) {
-                out = (sel4(ca, l) * de) * xe;
-            } else if constexpr (OP == EW_COPY3) {
-                out = (T)1.0 * xe;
-                if (a.y2 != nullptr)
-                    a.y2[e] = out;
-                if (a.y3 != nullptr)
-                    a.y3[e] = out;
-            } else { // EW_MX_XUPD
-                out = sel4(frozen, l) ? ze
-                                      : (T)sel4(ao, l) * (T)xo + (T)sel4(bo, l) * (T)yo +
-                                            (T)1.0 * ze;
-            }
-            a.z[e] = out;
-            if constexpr (ND == 1) {
-                const double wv = (a.w != nullptr && !copy) ? (double)we : (double)out;
-                acc_slot(dacc[0], l, (double)out * wv);
-            } else if constexpr (ND == 3) {
-                const double o = (double)out, wv = copy ? o : (double)we;
-                acc_slot(dacc[0], l, o * wv);
-                acc_slot(dacc[1], l, o * o);
-                acc_slot(dacc[2], l, wv * wv);
+#pragma unroll
+            for (int u = 0; u < K; u++) {
+                const uint32_t q = qb + u * BT;
+                if (q >= q1)
+                    break;
+                const uint32_t li = div_nv(q, nv);
+                const int l = (int)(q - li * nv);
+                const int64_t e = eb + q;
+                T out;
+                if constexpr (OP == EW_FILL) {
+                    out = a.c0;
+                } else if constexpr (OP == EW_AXPBY_Z) {
+                    out = keep ? ze[u] : sel4(ca, l) * xe[u] + sel4(cb, l) * ye[u];
+                    if (copy && a.y2 != nullptr)
+                        a.y2[e] = out;
+                } else if constexpr (OP == EW_AXPBYPCZ) {
+                    if (copy)
+                        out = xe[u];
+                    else
+                        out = (keep || sel4(frozen, l))
+                                  ? ze[u]
+                                  : sel4(ca, l) * xe[u] + sel4(cb, l) * ye[u] +
+                                        sel4(cc, l) * ze[u];
+                } else if constexpr (OP == EW_JACOBI0) {
+                    out = (sel4(ca, l) * de[u]) * xe[u];
+                } else if constexpr (OP == EW_COPY3) {
+                    out = (T)1.0 * xe[u];
+                    if (a.y2 != nullptr)
+                        a.y2[e] = out;
+                    if (a.y3 != nullptr)
+                        a.y3[e] = out;
+                } else { // EW_MX_XUPD
+                    out = sel4(frozen, l) ? ze[u]
+                                          : (T)sel4(ao, l) * (T)xo[u] +
+                                                (T)sel4(bo, l) * (T)yo[u] + (T)1.0 * ze[u];
+                }
+                a.z[e] = out;
+                if constexpr (ND == 1) {
+                    const double wv = (a.w != nullptr && !copy) ? (double)we[u] : (double)out;
+                    acc_slot(dacc[0], l, (double)out * wv);
+                } else if constexpr (ND == 3) {
+                    const double o = (double)out, wv = copy ? o : (double)we[u];
+                    acc_slot(dacc[0], l, o * wv);
+                    acc_slot(dacc[1], l, o * o);
+                    acc_slot(dacc[2], l, wv * wv);
+                }
             }
         }
         if constexpr (ND > 0)
