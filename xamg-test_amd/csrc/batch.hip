@@ -725,6 +725,7 @@ void Batch::set_tiles(int rows) {
     up(tsys, ts.data(), sizeof(int) * ts.size());
     up(trow, tr.data(), sizeof(uint32_t) * tr.size());
     up(stile, h_stile.data(), sizeof(int) * h_stile.size());
+    gen++; // a captured graph holds the old arrays' addresses
 }
 
 BLayout Batch::layout() const {
@@ -821,6 +822,7 @@ class BatchKrylov : public BatchKrylovBase {
     hipGraphExec_t gexec = nullptr;
     const double *g_x = nullptr, *g_b = nullptr;
     int g_k = 0, g_tiles = 0, g_grid = 0;
+    int g_gen = -1; // tile-array generation the graph's launches point into
 
     explicit BatchKrylov(Batch &b) : B(b) {}
     ~BatchKrylov() override {
@@ -1081,7 +1083,8 @@ class BatchKrylov : public BatchKrylovBase {
         init(x, b, st, L);
         // one graph of K iterations, rebuilt when the vectors or the launch shape change
         if (cfg.use_graph &&
-            (!gexec || g_x != x || g_b != b || g_k != K || g_tiles != B.ntiles || g_grid != cfg.grid)) {
+            (!gexec || g_x != x || g_b != b || g_k != K || g_tiles != B.ntiles ||
+             g_grid != cfg.grid || g_gen != B.gen)) {
             if (gexec) {
                 (void)hipGraphExecDestroy(gexec);
                 gexec = nullptr;
@@ -1098,6 +1101,7 @@ class BatchKrylov : public BatchKrylovBase {
             g_k = K;
             g_tiles = B.ntiles;
             g_grid = cfg.grid;
+            g_gen = B.gen;
         }
         hipEvent_t ev[2];
         XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));

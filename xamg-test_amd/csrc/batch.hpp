@@ -96,6 +96,7 @@ class Batch {
     // tiling (rebuilt when the tile size changes)
     int tile_rows = 0, ntiles = 0;
     DBuf tsys, trow, stile;
+    int gen = 0; // incremented whenever the tile arrays are reallocated
     std::vector<int> h_stile;
     std::unique_ptr<BatchKrylovBase> solver;
     double last_ms = 0;

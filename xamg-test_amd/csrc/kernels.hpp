@@ -28,16 +28,22 @@ enum Epi : int {
     // Horner steps after the first on the scaled operator D^{-1/2} A D^{-1/2}: x is used
     // as gathered (no column scaling), so the kernels carry no scaling code and need far
     // fewer registers (more waves in flight)
-    EPI_CHEB_S = 7,     // y = c*(b*d_i) + 1*acc
-    EPI_CHEB_LAST_S = 8 // as EPI_CHEB_S then y = 1*(y*d_i) [+ 1*xold]
+    EPI_CHEB_S = 7,      // y = c*(b*d_i) + 1*acc
+    EPI_CHEB_LAST_S = 8, // as EPI_CHEB_S then y = 1*(y*d_i) [+ 1*xold]
+    // first Horner step on the scaled operator (not also the last): x is the right-hand
+    // side b, gathered as c_{k-1} (b_j d_j), then as EPI_CHEB_S
+    EPI_CHEB_SF = 9
 };
 
 // Epilogue families of the Chebyshev steps.
 constexpr bool epi_cheb(int e) {
-    return e == EPI_CHEB || e == EPI_CHEB_LAST || e == EPI_CHEB_S || e == EPI_CHEB_LAST_S;
+    return e == EPI_CHEB || e == EPI_CHEB_LAST || e == EPI_CHEB_S || e == EPI_CHEB_LAST_S ||
+           e == EPI_CHEB_SF;
 }
 constexpr bool epi_cheb_last(int e) { return e == EPI_CHEB_LAST || e == EPI_CHEB_LAST_S; }
-constexpr bool epi_cheb_scaled(int e) { return e == EPI_CHEB_S || e == EPI_CHEB_LAST_S; }
+constexpr bool epi_cheb_scaled(int e) {
+    return e == EPI_CHEB_S || e == EPI_CHEB_LAST_S || e == EPI_CHEB_SF;
+}
 
 // Elementwise ops on [n][nv] vectors with per-RHS coefficients (nullptr = 1).
 enum Ew : int {

@@ -472,12 +472,14 @@ class Chebyshev : public Solver<T> {
                 s.d = d;
                 s.dcol = d;
                 s.coef = c[q];
-                // steps after the first on the scaled operator gather x as it is
+                // steps on the scaled operator have compile-time epilogues: the first
+                // (when not also the last) gathers b scaled by d, the others x as it is
                 const bool lean = !first && hprec == Prec::F32S;
+                const bool lean_first = first && hprec == Prec::F32S;
                 if (q > 0) {
                     s.y = t2;
-                    spmv<T>(*A, hprec, lean ? EPI_CHEB_S : EPI_CHEB, 0, nv, s, red_args(H),
-                            FinArgs<T>{});
+                    const int epi = lean ? EPI_CHEB_S : lean_first ? EPI_CHEB_SF : EPI_CHEB;
+                    spmv<T>(*A, hprec, epi, 0, nv, s, red_args(H), FinArgs<T>{});
                     std::swap(t1, t2);
                 } else {
                     s.y = x;

@@ -210,7 +210,16 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
 #pragma unroll
         for (int u = 0; u < U; u++)
             ld_vec<T, VPT>(a.x + (int64_t)c[u] * NV + l0, xv[u]);
-        if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
+        if constexpr (EPI == EPI_CHEB_SF) {
+            // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const T dc = a.dcol[c[u]];
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    xv[u][v] = a.xcoef * (xv[u][v] * dc);
+            }
+        } else if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
             if (a.x_is_rhs && a.prescaled) {
                 // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
 #pragma unroll
@@ -568,6 +577,7 @@ void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const Fin
     XAMG_SPMV_CASE(EPI_CHEB_LAST, 0)
     XAMG_SPMV_CASE(EPI_CHEB_S, 0)
     XAMG_SPMV_CASE(EPI_CHEB_LAST_S, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_SF, 0)
     if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value) {
         XAMG_SPMV_CASE(EPI_RESID_MX, 0)
         XAMG_SPMV_CASE(EPI_RESID_MX, 1)
