@@ -478,8 +478,15 @@ void launch_finish(int nd, const RedArgs &r, const FinArgs<T> &f, int G, hipStre
 // Element-wise kernels (row ranges, one lane per (row, RHS) element).  Every operand of
 // an element is loaded before the first use of any of them, without branches between
 // the loads, so each element costs one round trip to memory; the rare keep / copy /
-// freeze cases of the two-term update take a separate loop.
+// freeze cases of the two-term update take a separate loop.  The streamed operands of
+// the update kernels are loaded non-temporally (ld_stream), so that they do not displace
+// the matrix and the gathered vectors of the SpMVs from the last-level cache.
 // ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
 template <typename T, int NV, int OP, int ND>
 __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
     if (r.halt != nullptr && *r.halt) {
@@ -547,13 +554,13 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
         auto run = [&](auto general) {
             constexpr bool GEN = decltype(general)::value;
             for (int64_t e = e0 + tid; e < e1; e += kBlock) {
-                const T xe = a.x[e], ye = a.y[e];
+                const T xe = ld_stream(a.x + e), ye = ld_stream(a.y + e);
                 T we = (T)0, w2e = (T)0, ze = (T)0;
                 if constexpr (OP == EW_RB_R) {
                     we = a.w[e];
                     w2e = a.w2[e];
                 } else if constexpr (ND > 0) {
-                    we = wp[e];
+                    we = ld_stream(wp + e);
                 }
                 if constexpr (GEN)
                     ze = a.z[e];
@@ -581,7 +588,7 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
             if constexpr (OP == EW_FILL) {
                 out = a.c0;
             } else if constexpr (OP == EW_AXPBYPCZ) {
-                const T xe = a.x[e], ye = a.y[e], ze = a.z[e];
+                const T xe = ld_stream(a.x + e), ye = ld_stream(a.y + e), ze = ld_stream(a.z + e);
                 const T upd = ca * xe + cb * ye + cc * ze;
                 out = copy ? xe : ((frozen || keep) ? ze : upd);
             } else if constexpr (OP == EW_JACOBI0) {
@@ -613,8 +620,9 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
             } else if constexpr (OP == EW_DOTS) {
                 out = a.x[e];
             } else { // EW_MX_XUPD: FP64 accumulation of FP32 directions
-                const T ze = a.z[e];
-                const T xe = (T)((const float *)a.xo)[e], ye = (T)((const float *)a.yo)[e];
+                const T ze = ld_stream(a.z + e);
+                const T xe = (T)ld_stream((const float *)a.xo + e);
+                const T ye = (T)ld_stream((const float *)a.yo + e);
                 const T upd = ao * xe + bo * ye + (T)1.0 * ze;
                 out = frozen ? ze : upd;
             }

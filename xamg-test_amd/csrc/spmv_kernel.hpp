@@ -175,26 +175,31 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
             acc[v] = (T)0;
     }
     // the epilogue's own row operands are loaded before the gathers so their
-    // latency overlaps the nonzero loop
+    // latency overlaps the nonzero loop.  They are read once per launch and not again
+    // before other kernels have streamed more than the last-level cache holds, so they
+    // are loaded non-temporally: they do not displace the matrix and the vector the next
+    // kernel gathers from the 256 MB last-level cache (a first Chebyshev step's b is its
+    // gathered x and keeps ordinary loads).
     constexpr bool kNeedB = EPI != EPI_AXY && EPI != EPI_AXPY;
+    constexpr bool NTB = EPI != EPI_CHEB_SF;
     constexpr bool kNeedD = EPI == EPI_JACOBI || epi_cheb(EPI);
     T pb[VPT], pw[VPT], px[VPT];
     T pdi = (T)0;
     if constexpr (kNeedB)
-        ld_vec<T, VPT, NT>(a.b + e, pb);
+        ld_vec<T, VPT, NTB>(a.b + e, pb);
     if constexpr (kNeedD)
         pdi = a.d[i];
     if constexpr (EPI == EPI_JACOBI) {
-        ld_vec<T, VPT, NT>(a.xin + e, px);
+        ld_vec<T, VPT, true>(a.xin + e, px);
     } else if constexpr (epi_cheb_last(EPI)) {
         if (a.xin != nullptr)
-            ld_vec<T, VPT, NT>(a.xin + e, px);
+            ld_vec<T, VPT, true>(a.xin + e, px);
     }
     if constexpr (ND == 3) {
-        ld_vec<T, VPT, NT>(a.w + e, pw);
+        ld_vec<T, VPT, true>(a.w + e, pw);
     } else if constexpr (ND == 1) {
         if (a.w != nullptr)
-            ld_vec<T, VPT, NT>(a.w + e, pw);
+            ld_vec<T, VPT, true>(a.w + e, pw);
     }
     for (uint32_t j = j0; j < j1; j += U) {
         uint32_t c[U];
