@@ -203,9 +203,18 @@ def other_configs(X, tol):
     mgr = json.loads(json.dumps(mg))
     mg["preconditioner"] = dict(mgb)
     mgr["preconditioner"] = dict(mgb, mg_reduced_precision=True)
+    # FP32 PBiCGStab recurrences and an FP32 cycle inside continued FP64 refinement; the
+    # coarsest level solved with the FP64 inverse
+    mgc = {"meta_solver": dict(mixed_meta),
+           "solver": {"method": "PBiCGStab", "rel_tolerance": 1e-4, "abs_tolerance": 0.0,
+                      "max_iters": 200},
+           "preconditioner": dict(mgb),
+           "pre_smoother": {"method": "Chebyshev"}, "post_smoother": {"method": "Chebyshev"},
+           "coarse_grid_solver": {"method": "Direct", "fp64_solve": True}}
     c = np.cos(np.pi / 193)
     run_pair("7pt-poisson 192^3 nv=16 pbicgstab+amg", row, col, val, 16, 4,
-             [("fp64_levels", mg), ("fp32_levels", mgr)], eigs=(1 - c, 1 + c), hier=H)
+             [("fp64_levels", mg), ("fp32_levels", mgr), ("fp32_continued_fp32_cycle", mgc)],
+             eigs=(1 - c, 1 + c), hier=H)
     out["7pt-poisson 192^3 nv=16 pbicgstab+amg"]["host_setup_s"] = setup_s
     out["7pt-poisson 192^3 nv=16 pbicgstab+amg"]["levels"] = H.levels()
     H.destroy()

@@ -114,3 +114,61 @@ def test_cfg_mg_192cube_nv16(xamg):
     assert rel.max() <= 1.5e-10, rel.max()
     for o in (S, P, x, bv, A_):
         o.destroy()
+
+
+def test_direct_fp64_solve_in_fp32_refinement(xamg):
+    """Direct with fp64_solve inside restarted FP32/FP64 refinement: the FP32 inner solve
+    applies the FP64 inverse with an FP64 sum, so each correction is exact to FP32 rounding
+    of its result and the refinement needs fewer outer iterations than with the FP32
+    inverse; both reach the FP64-verified tolerance."""
+    nx, nv = 9, 2
+    A = poisson7_scipy(nx)
+    A = (A + 0.001 * A @ A).tocsr()  # a wider spectrum than the Laplacian alone
+    A.sort_indices()
+    row, col, val = A.indptr, A.indices, A.data
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 6)
+    outer = {}
+    for fp64 in (False, True):
+        params = {"meta_solver": {"method": "IterativeRefinement",
+                                  "subsolver_reduced_precision": True, "rel_tolerance": 1e-13,
+                                  "abs_tolerance": 0.0, "max_iters": 20},
+                  "solver": {"method": "Direct", "fp64_solve": fp64}}
+        out = xamg.solve_system(row, col, val, b, nv, params)
+        assert all(out["converged"])
+        X_ = out["x"].reshape(n, nv)
+        B = b.reshape(n, nv)
+        for l in range(nv):
+            assert np.linalg.norm(B[:, l] - A @ X_[:, l]) <= 1e-13 * np.linalg.norm(B[:, l])
+        outer[fp64] = out["iters"]
+    assert outer[True] <= outer[False], outer
+    assert outer[True] <= 3, outer
+
+
+def test_fp32_cycle_in_continued_refinement(xamg):
+    """FP32 PBiCGStab recurrences preconditioned by an FP32 V(1,1) cycle (Chebyshev
+    smoothers, coarsest level with the FP64 inverse) inside continued FP64 refinement:
+    FP64-verified residual, error against a direct solve, inner iterations close to the
+    FP64 solver's."""
+    nx, nv = 16, 4
+    A = poisson7_scipy(nx)
+    row, col, val = A.indptr, A.indices, A.data
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 8)
+    mg = _mg_block()
+    p64 = _params("PBiCGStab", mg, {"method": "Chebyshev"})
+    pmx = _params("PBiCGStab", mg, {"method": "Chebyshev"}, tol=1e-4)
+    pmx["meta_solver"] = {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
+                          "rel_tolerance": 1e-10, "abs_tolerance": 0.0, "refinement": "continued"}
+    pmx["coarse_grid_solver"] = {"method": "Direct", "fp64_solve": True}
+    H = X.Hierarchy.build_csr(X.Params({"preconditioner": mg}), row, col, val)
+    eigs = poisson_eigs(nx)
+    ref = xamg.solve_system(row, col, val, b, nv, p64, eigs=eigs, hierarchy=H)
+    mx = xamg.solve_system(row, col, val, b, nv, pmx, eigs=eigs, hierarchy=H)
+    assert all(ref["converged"]) and all(mx["converged"])
+    X_ = mx["x"].reshape(n, nv)
+    B = b.reshape(n, nv)
+    for l in range(nv):
+        assert np.linalg.norm(B[:, l] - A @ X_[:, l]) <= 1e-10 * np.linalg.norm(B[:, l])
+    assert np.linalg.norm(mx["x"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
+    assert mx["sub_iters"] <= 1.5 * ref["iters"] + 3, (mx["sub_iters"], ref["iters"])

@@ -637,10 +637,12 @@ __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T
 
 // Dense GEMV for the coarse Direct solve (dense_matvec.inl:100-117): one lane per
 // (row, RHS) sums the columns in ascending order; A is stored column-major (A^T row-major)
-// so the lanes of a wave read consecutive rows of one column.
-template <typename T>
+// so the lanes of a wave read consecutive rows of one column.  TA is the precision of the
+// inverse and of the sum, T that of the vectors (an FP32 cycle can solve its coarsest
+// level with the FP64 inverse).
+template <typename TA, typename T>
 __global__ void __launch_bounds__(kBlock)
-    k_dense_mv(const T *__restrict__ AT, int64_t n, const T *__restrict__ x, T *__restrict__ y,
+    k_dense_mv(const TA *__restrict__ AT, int64_t n, const T *__restrict__ x, T *__restrict__ y,
                int nv, const int *halt) {
     if (halt != nullptr && *halt)
         return;
@@ -649,10 +651,10 @@ __global__ void __launch_bounds__(kBlock)
         return;
     const int64_t i = e / nv;
     const int l = (int)(e % nv);
-    T s = 0;
+    TA s = 0;
     for (int64_t j = 0; j < n; j++)
-        s = s + AT[j * n + i] * x[j * nv + l];
-    y[e] = s;
+        s = s + AT[j * n + i] * (TA)x[j * nv + l];
+    y[e] = (T)s;
 }
 
 // Block-hybrid SGS.  One workgroup per core block; the NV lanes of a row relax the
@@ -809,14 +811,14 @@ void launch_ew(int op, int ndots, int nv, const EwArgs<T> &a, const RedArgs &r,
     XAMG_HIP_CHECK(hipGetLastError());
 }
 
-template <typename T>
-void launch_dense_mv(const T *at, int64_t n, const T *x, T *y, int nv, const int *halt,
+template <typename TA, typename T>
+void launch_dense_mv(const TA *at, int64_t n, const T *x, T *y, int nv, const int *halt,
                      hipStream_t s) {
     if (n <= 0)
         return;
     int64_t tot = n * nv;
-    hipLaunchKernelGGL(k_dense_mv<T>, dim3((unsigned)((tot + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, at, n, x, y, nv, halt);
+    hipLaunchKernelGGL((k_dense_mv<TA, T>), dim3((unsigned)((tot + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, at, n, x, y, nv, halt);
     XAMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -909,10 +911,12 @@ template void launch_ew<double>(int, int, int, const EwArgs<double> &, const Red
                                 const FinArgs<double> &, hipStream_t);
 template void launch_ew<float>(int, int, int, const EwArgs<float> &, const RedArgs &,
                                const FinArgs<float> &, hipStream_t);
-template void launch_dense_mv<double>(const double *, int64_t, const double *, double *, int,
-                                      const int *, hipStream_t);
-template void launch_dense_mv<float>(const float *, int64_t, const float *, float *, int,
-                                     const int *, hipStream_t);
+template void launch_dense_mv<double, double>(const double *, int64_t, const double *, double *,
+                                              int, const int *, hipStream_t);
+template void launch_dense_mv<float, float>(const float *, int64_t, const float *, float *, int,
+                                            const int *, hipStream_t);
+template void launch_dense_mv<double, float>(const double *, int64_t, const float *, float *, int,
+                                             const int *, hipStream_t);
 template void launch_hsgs<double, double>(const uint32_t *, const uint32_t *, const double *,
                                           const double *, int64_t, int, const double *, double *,
                                           double *, const double *, const double *, int,

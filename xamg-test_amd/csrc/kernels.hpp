@@ -241,8 +241,8 @@ void launch_finish(int nd, const RedArgs &r, const FinArgs<T> &f, int G, hipStre
 
 // Dense GEMV y = A x for the coarse-grid Direct solver (dense_matvec.inl:100-117);
 // `at` holds A column-major.
-template <typename T>
-void launch_dense_mv(const T *at, int64_t n, const T *x, T *y, int nv, const int *halt,
+template <typename TA, typename T>
+void launch_dense_mv(const TA *at, int64_t n, const T *x, T *y, int nv, const int *halt,
                      hipStream_t s);
 
 // Block-hybrid symmetric Gauss-Seidel sweep (csr_gs.inl:32-134 + blas2.h:225-263):

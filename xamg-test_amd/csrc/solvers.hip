@@ -235,14 +235,27 @@ class Identity : public Solver<T> {
 
 template <typename T>
 class Direct : public Solver<T> {
+    // fp64_solve (extension key): an FP32 solver applies the FP64 inverse with an FP64
+    // sum, as the coarsest level of an FP32 multigrid cycle inside FP64 refinement
+    bool fp64 = false;
+
   public:
     void setup() override {
         Solver<T>::setup();
-        this->A->template dense_inverse<T>();
+        fp64 = std::is_same<T, float>::value && this->prm.has("fp64_solve") &&
+               this->prm.get_bool("fp64_solve");
+        if (fp64)
+            this->A->template dense_inverse<double>();
+        else
+            this->A->template dense_inverse<T>();
     }
     void apply(T *x, bool, const T *b, const int *halt) override {
-        launch_dense_mv<T>(this->A->template dense_inverse<T>(), this->A->n, b, x, this->nv, halt,
-                           Runtime::get().stream);
+        if (fp64)
+            launch_dense_mv<double, T>(this->A->template dense_inverse<double>(), this->A->n, b, x,
+                                       this->nv, halt, Runtime::get().stream);
+        else
+            launch_dense_mv<T, T>(this->A->template dense_inverse<T>(), this->A->n, b, x, this->nv,
+                                  halt, Runtime::get().stream);
     }
 };
 
