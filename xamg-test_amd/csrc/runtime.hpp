@@ -133,6 +133,7 @@ struct DevMatrix {
         std::vector<int> issued;
         std::vector<Sample> pending;
         int next = 0;
+        int dropped = 0; // samples of launches that did almost nothing
         bool done = false;
     };
     std::map<int, Online> online;
