@@ -379,11 +379,8 @@ __device__ inline bool fin_shadow_plan(const FinArgs<T> &f, uint32_t *lds, FinSh
 
 template <typename T, int NV, int ND>
 __global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, int G) {
-    if (r.halt != nullptr && *r.halt)
-        return;
-    if (r.gate != nullptr && !*r.gate)
-        return;
-    if (r.set_halt_if_done && r.cs != nullptr && r.cs->done)
+    const StartFlags fl = start_flags(r);
+    if (fl.halt || !fl.gate || fl.done)
         return;
     __shared__ double sums[ND * NV];
     __shared__ uint64_t shadow_mem[(FinShadow::kWords + 1) / 2];
@@ -489,19 +486,22 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 
 template <typename T, int NV, int OP, int ND>
 __global__ void __launch_bounds__(kBlock) k_ew(EwArgs<T> a, RedArgs r, FinArgs<T> f, int remap) {
-    if (r.halt != nullptr && *r.halt) {
+    // launch flags and the keep / copy flags of the update, loaded together
+    const StartFlags fl = start_flags(r);
+    const int keep_v = *(a.keep_if != nullptr ? a.keep_if : &kStartFlagConst[0]);
+    const int copy_v = *(a.copy_if != nullptr ? a.copy_if : &kStartFlagConst[0]);
+    if (fl.halt)
         return;
-    }
-    if (r.set_halt_if_done && r.cs != nullptr && r.cs->done) {
+    if (fl.done) {
         if (blockIdx.x == 0 && threadIdx.x == 0)
             r.cs->halt = 1;
         return;
     }
-    if (r.gate != nullptr && !*r.gate)
+    if (!fl.gate)
         return;
     using O = typename std::conditional<std::is_same<T, double>::value, float, double>::type;
-    const bool keep = a.keep_if != nullptr && *a.keep_if;
-    const bool copy = a.copy_if != nullptr && *a.copy_if;
+    const bool keep = keep_v != 0;
+    const bool copy = copy_v != 0;
     const int tid = threadIdx.x;
     const int l = tid % NV;
     const int G = gridDim.x;
