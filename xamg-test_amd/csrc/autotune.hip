@@ -197,7 +197,8 @@ TuneResult run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
     // tuning; the XCD mapping and the staging mode do not change the order.
     const std::vector<int> vpts = nd > 0 ? std::vector<int>{dot_vpt(A, nv)} : vpt_options(nv);
     const std::vector<int> grids =
-        nd > 0 ? std::vector<int>{launch_cfg().grid_red} : std::vector<int>{1024, 2048, 4096, 8192};
+        nd > 0 ? std::vector<int>{launch_cfg().grid_red}
+               : std::vector<int>{1024, 2048, 4096, 8192, 16384};
     std::vector<int> modes = {0, 1, 2, 3};
     if (A.col16.p != nullptr) {
         modes.push_back(4);
