@@ -280,8 +280,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build targets one GPU; --gpus N with N > 1 is not supported "
-              "(the path would shard by z-slab row blocks, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: the solver runs on one GPU; --gpus N with N > 1 is not supported "
+              "(the path would shard by z-slab row blocks with a halo exchange per SpMV "
+              "and one small all-reduce per dot)", file=sys.stderr)
         sys.exit(2)
 
     import pyxamg as X
