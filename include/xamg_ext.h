@@ -84,6 +84,13 @@ int xamgx_solver_last_ms(XAMG_solver_id sid, double *ms);
 int xamgx_solver_get_history(XAMG_solver_id sid, double *out, int cap_rows, int *nrows);
 /* Squared initial / final residual norms per RHS (convergence.h res0/res). */
 int xamgx_solver_get_residuals(XAMG_solver_id sid, double *res0, double *res);
+/* Residual replacement record of the last solve (convergence_details RR,
+ * convergence.h:269-302): true residuals computed at the RR_stepping checkpoints, and
+ * whether a recurrence/true deviation beyond RR_deviation ended the solve.  For
+ * IterativeRefinement, summed over its inner solves (stopped = inner solves ended so).
+ * Only the Krylov solvers compute them; elsewhere the checked residual is the true one.
+ * NULL skips an output. */
+int xamgx_solver_get_rr_info(XAMG_solver_id sid, int *checks, int *stopped);
 
 /* ---- configuration ---- */
 /* Batch of iterations between host polls of the device halt flag, HIP-graph replay

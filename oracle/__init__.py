@@ -49,13 +49,15 @@ Node._fields_ = [("method", C.c_int), ("max_iters", C.c_int), ("abs_tol", C.c_do
                  ("max_eig", C.c_double), ("gs_blocks", C.c_int), ("mg_cycle", C.c_int),
                  ("reduced", C.c_int), ("precond", C.POINTER(Node)), ("pre", C.POINTER(Node)),
                  ("post", C.POINTER(Node)), ("coarse", C.POINTER(Node)),
-                 ("hier", C.POINTER(Hier)), ("nchunks", C.c_int)]
+                 ("hier", C.POINTER(Hier)), ("nchunks", C.c_int),
+                 ("rr", C.c_int), ("rr_stepping", C.c_double), ("rr_deviation", C.c_double)]
 
 
 class Result(C.Structure):
     _fields_ = [("iters", C.c_int), ("sub_iters", C.c_int), ("converged", C.c_int * MAX_NV),
                 ("res0", C.c_double * MAX_NV), ("res", C.c_double * MAX_NV),
-                ("hist", C.c_void_p), ("hist_cap", C.c_int), ("nhist", C.c_int)]
+                ("hist", C.c_void_p), ("hist_cap", C.c_int), ("nhist", C.c_int),
+                ("rr_checks", C.c_int), ("rr_stopped", C.c_int)]
 
 
 def build():
@@ -252,7 +254,13 @@ class Tree:
         n.max_iters = int(self._get(role, "max_iters"))
         n.abs_tol = float(self._get(role, "abs_tolerance"))
         n.rel_tol = float(self._get(role, "rel_tolerance"))
-        n.conv_check = 0 if "nocheck" in self._get(role, "convergence_details") else 1
+        details = set(str(self._get(role, "convergence_details")).split(","))
+        n.conv_check = 0 if "nocheck" in details else 1
+        # residual replacement (stats.h:111-122, xamg_params_details.h:184-187)
+        n.rr = int("RR" in details and n.conv_check == 1)
+        rc = self.params.get("residual_control", {})
+        n.rr_stepping = float(rc.get("RR_stepping", 0.1))
+        n.rr_deviation = float(rc.get("RR_deviation", 2.0))
         n.relax = float(self._get(role, "relax_factor"))
         n.poly_order = int(p.get("polynomial_order", 2))
         n.spec_frac = float(p.get("spectrum_fraction", 0.3))
@@ -327,6 +335,7 @@ class Solve:
             "converged": [bool(r.converged[l]) for l in range(nv)],
             "res0": np.array(r.res0[:nv]), "res": np.array(r.res[:nv]),
             "hist": self.hist[: r.nhist * nv].reshape(r.nhist, nv).copy(),
+            "rr_checks": r.rr_checks, "rr_stopped": bool(r.rr_stopped),
         }
 
 

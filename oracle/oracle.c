@@ -156,6 +156,8 @@ void *orc_setup(const orc_node *node, const orc_csr *A, int nv, int single) {
 static void fill_result_d(const conv_t_d *cv, int nv, orc_result *res) {
     res->iters = cv->iter;
     res->sub_iters = cv->sub_iter;
+    res->rr_checks = cv->rr_checks;
+    res->rr_stopped = cv->rr_stopped;
     for (int l = 0; l < nv; l++) {
         res->converged[l] = cv->status[l];
         res->res0[l] = cv->res0[l];
@@ -166,6 +168,8 @@ static void fill_result_d(const conv_t_d *cv, int nv, orc_result *res) {
 static void fill_result_f(const conv_t_f *cv, int nv, orc_result *res) {
     res->iters = cv->iter;
     res->sub_iters = cv->sub_iter;
+    res->rr_checks = cv->rr_checks;
+    res->rr_stopped = cv->rr_stopped;
     for (int l = 0; l < nv; l++) {
         res->converged[l] = cv->status[l];
         res->res0[l] = cv->res0[l];

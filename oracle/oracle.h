@@ -126,6 +126,10 @@ struct orc_node {
     orc_node *pre, *post, *coarse; /* MultiGrid smoothers (same for all levels) and coarse solver */
     const orc_hierarchy *hier;
     int nchunks; /* dot-product chunking (cores of the modelled rank) */
+    /* residual replacement: convergence_details RR with the residual_control block's
+     * RR_stepping / RR_deviation (xamg_params_details.h:104-105, 184-187) */
+    int rr;
+    double rr_stepping, rr_deviation;
 };
 
 /* Convergence record of the top solver (stats.h:120-148, convergence.h:49-303). */
@@ -140,6 +144,9 @@ typedef struct {
     double *hist;
     int hist_cap;
     int nhist;
+    /* residual replacement: true residuals computed, 1 if a deviation ended the solve */
+    int rr_checks;
+    int rr_stopped;
 } orc_result;
 
 /* Solve A x = b in FP64 (x in/out, [n][nv] interleaved). */

@@ -154,6 +154,11 @@ typedef struct {
     int iter, sub_iter;
     int nv;
     orc_result *rec;
+    /* residual replacement (convergence_details RR, residual_control block) */
+    int rr;
+    double rr_stepping, rr_deviation; /* float32_t in convergence.h:61-62 */
+    double rr_tol;
+    int rr_checks, rr_stopped;
 } SFX(conv_t);
 
 /* convergence.h:173-195 init(conv_vector) / init() */
@@ -163,6 +168,8 @@ static void SFX(conv_init)(SFX(conv_t) * c, const F *iconv) {
         c->res0[l] = c->res[l] = 0.0;
         c->status[l] = iconv ? (fabs(1.0 - (double)iconv[l]) > 1e-10) : 0;
     }
+    c->rr_tol = c->rr_stepping; /* convergence.h:184-186, 193-195 */
+    c->rr_checks = c->rr_stopped = 0;
 }
 
 static void SFX(conv_record)(SFX(conv_t) * c) {
@@ -228,6 +235,7 @@ struct SFX(rt) {
     F cheb[4];
     F *buf[8];
     int nbuf;
+    F *rrbuf; /* residual replacement: true residual (stats.h:58-59 rr_temp_vector) */
     SFX(conv_t) conv;
     SFX(rt) * precond;
     /* MultiGrid */
@@ -283,6 +291,48 @@ static void SFX(norm2)(SFX(rt) * rt, const F *r, F *res) {
 
 static void SFX(solve)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv);
 
+/* stats.h:151-160 rr_is_converged with convergence.h:269-302 rr_check and
+ * rr_deviation_check.  Once the largest relative recurrence residual sqrt(res/res0)
+ * over every RHS (a converged one at its last value) has fallen to rr_tol, rr_tol
+ * becomes that value times RR_stepping, the true residual b - A x is computed
+ * (solver.h:183-190) and its squared norm taken; a recurrence/true ratio of the norms
+ * outside [1/RR_deviation, RR_deviation] for any RHS ends the solve: the check reports
+ * convergence and the RHS statuses stay as they are. */
+static int SFX(rr_is_converged)(SFX(rt) * rt, const F *x, int xzero, const F *b) {
+    SFX(conv_t) *c = &rt->conv;
+    if (!c->rr || !c->conv_check)
+        return 0;
+    double maxrel = 0.0;
+    for (int l = 0; l < c->nv; l++) {
+        const double ratio = sqrt(c->res[l] / c->res0[l]);
+        maxrel = maxrel < ratio ? ratio : maxrel; /* std::max(maxrel, ratio) */
+    }
+    if (maxrel > c->rr_tol)
+        return 0;
+    c->rr_tol = maxrel * c->rr_stepping;
+    c->rr_checks++;
+    F tn[ORC_MAX_NV];
+    SFX(residual)(&rt->A, x, xzero, b, rt->rrbuf, rt->nv);
+    SFX(norm2)(rt, rt->rrbuf, tn);
+    for (int l = 0; l < c->nv; l++) {
+        const double ratio = sqrt(c->res[l] / (double)tn[l]);
+        if (ratio > c->rr_deviation || ratio < 1.0 / c->rr_deviation) {
+            c->rr_stopped = 1;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* stats.h:165-181 internal_is_converged: the check, then residual replacement */
+static int SFX(conv_is_converged_rr)(SFX(rt) * rt, const F *rnorm, F *iconv, const F *x, int xzero,
+                                     const F *b) {
+    int flag = SFX(conv_is_converged)(&rt->conv, rnorm, iconv);
+    if (!flag)
+        flag = SFX(rr_is_converged)(rt, x, xzero, b);
+    return flag;
+}
+
 /* ---- BiCGStab bicgstab.inl:43-197 / PBiCGStab pbicgstab.inl:47-207 ---- */
 static void SFX(solve_bicgstab)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv_in,
                                 int precond) {
@@ -307,7 +357,7 @@ static void SFX(solve_bicgstab)(SFX(rt) * rt, F *x, int xzero, const F *b, F *ic
     SFX(norm2)(rt, r0, rho);
     if (c->conv_check) {
         SFX(conv_set_initial)(c, rho);
-        if (SFX(conv_is_converged)(c, rho, iconv))
+        if (SFX(conv_is_converged_rr)(rt, rho, iconv, x, xzero, b))
             return;
     }
     memcpy(r, r0, sizeof(F) * N);
@@ -355,7 +405,7 @@ static void SFX(solve_bicgstab)(SFX(rt) * rt, F *x, int xzero, const F *b, F *ic
             res[l] = omega[l] * s2[l];
             res[l] = (F)1.0 * s4[l] + (F)(-1.0) * res[l];
         }
-        if (SFX(conv_is_converged)(c, res, iconv))
+        if (SFX(conv_is_converged_rr)(rt, res, iconv, x, 0, b))
             return;
         for (int l = 0; l < nv; l++)
             omega[l] *= (F)-1.0;
@@ -399,7 +449,7 @@ static void SFX(solve_pcg)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv_i
     if (c->conv_check) {
         SFX(norm2)(rt, r, rn);
         SFX(conv_set_initial)(c, rn);
-        if (SFX(conv_is_converged)(c, rn, iconv))
+        if (SFX(conv_is_converged_rr)(rt, rn, iconv, x, xzero, b))
             return;
     }
     /* z = M^{-1} r from a zero-flagged z; p = z */
@@ -438,7 +488,7 @@ static void SFX(solve_pcg)(SFX(rt) * rt, F *x, int xzero, const F *b, F *iconv_i
             for (int l = 0; l < nv; l++)
                 p[i * nv + l] = (F)1.0 * z[i * nv + l] + beta[l] * p[i * nv + l];
         SFX(orc_dot)(r, r, n, nv, rt->node->nchunks, rn);
-        if (SFX(conv_is_converged)(c, rn, iconv))
+        if (SFX(conv_is_converged_rr)(rt, rn, iconv, x, 0, b))
             return;
     }
 }
@@ -462,7 +512,7 @@ static void SFX(solve_rbicgstab)(SFX(rt) * rt, F *x, int xzero, const F *b, F *i
     SFX(norm2)(rt, r0, rho);
     if (c->conv_check) {
         SFX(conv_set_initial)(c, rho);
-        if (SFX(conv_is_converged)(c, rho, iconv))
+        if (SFX(conv_is_converged_rr)(rt, rho, iconv, x, xzero, b))
             return;
     }
     memcpy(r, r0, sizeof(F) * n * nv);
@@ -525,7 +575,7 @@ static void SFX(solve_rbicgstab)(SFX(rt) * rt, F *x, int xzero, const F *b, F *i
             for (int l = 0; l < nv; l++)
                 x[i * nv + l] = ac[l] * v_[i * nv + l] + oc[l] * t_[i * nv + l] +
                                 (F)1.0 * x[i * nv + l];
-        if (SFX(conv_is_converged)(c, res, iconv))
+        if (SFX(conv_is_converged_rr)(rt, res, iconv, x, 0, b))
             return;
         for (int l = 0; l < nv; l++) { /* 21-22 */
             rho[l] = omega[l] * psi[l];
@@ -901,9 +951,13 @@ static void solve_ir(rt_d *rt, double *x, int xzero, const double *b) {
             for (int64_t i = 0; i < n * nv; i++)
                 xc[i] = (double)rt->ix32[i];
             c->sub_iter += rt->inner32->conv.iter;
+            c->rr_checks += rt->inner32->conv.rr_checks; /* the inner solves' RR record */
+            c->rr_stopped += rt->inner32->conv.rr_stopped;
         } else {
             solve_d(rt->precond, xc, 1, r, NULL);
             c->sub_iter += rt->precond->conv.iter;
+            c->rr_checks += rt->precond->conv.rr_checks;
+            c->rr_stopped += rt->precond->conv.rr_stopped;
         }
         for (int64_t i = 0; i < n * nv; i++)
             x[i] = 1.0 * xc[i] + 1.0 * x[i];
@@ -1007,6 +1061,11 @@ SFX(rt) * SFX(rt_build)(const orc_node *node, const orc_csr *A, int nv, int leve
     rt->conv.conv_check = node->conv_check;
     rt->conv.abs_tol = node->abs_tol;
     rt->conv.rel_tol = node->rel_tol;
+    rt->conv.rr = node->rr;
+    rt->conv.rr_stepping = (double)(float)node->rr_stepping;
+    rt->conv.rr_deviation = (double)(float)node->rr_deviation;
+    if (node->rr)
+        rt->rrbuf = SFX(vnew)(A->nrows, nv);
     rt->nbuf = SFX(nbufs)(node->method);
     for (int k = 0; k < rt->nbuf; k++)
         rt->buf[k] = SFX(vnew)(rt->n, nv);

@@ -45,13 +45,17 @@ def _hist_dev(a, b, nv):
 
 
 def _compare(gpu, orc, nv, perturbed=None):
-    """Iterations within 2, same convergence flags, and a residual history that stays as
-    close to the oracle's as the oracle itself stays under a change of dot-product
-    summation order: `perturbed` holds the oracle rerun with 2/3/7/16/64-chunk dots (the
-    reference on ranks of that many cores).  Per check, the GPU deviation must not
-    exceed 10x the running maximum of those deviations (plus 1e-10), up to the check
-    where that maximum reaches CHAOS."""
-    assert abs(gpu["iters"] - orc["iters"]) <= 2, (gpu["iters"], orc["iters"])
+    """Iterations within 2 of the oracle's, same convergence flags, and a residual
+    history that stays as close to the oracle's as the oracle itself stays under a
+    change of dot-product summation order: `perturbed` holds the oracle rerun with
+    2/3/7/16/64-chunk dots (the reference on ranks of that many cores).  The oracle's
+    iteration count is the range over those summation orders (BiCGStab's count can move
+    by a few iterations with the rounding of its dots alone).  Per check, the GPU
+    deviation must not exceed 10x the running maximum of those deviations (plus 1e-10),
+    up to the check where that maximum reaches CHAOS."""
+    runs = [] if perturbed is None else (perturbed if isinstance(perturbed, list) else [perturbed])
+    its = [orc["iters"]] + [r["iters"] for r in runs]
+    assert min(its) - 2 <= gpu["iters"] <= max(its) + 2, (gpu["iters"], its)
     assert gpu["converged"] == orc["converged"]
     hg, ho = gpu["hist"], orc["hist"]
     assert min(len(hg), len(ho)) >= 2
@@ -60,7 +64,6 @@ def _compare(gpu, orc, nv, perturbed=None):
         k = min(len(dg), 5)
         assert np.nanmax(dg[:k]) <= 1e-8
         return
-    runs = perturbed if isinstance(perturbed, list) else [perturbed]
     k = min([len(dg)] + [len(r["hist"]) for r in runs])
     dp = np.zeros((k, nv))
     for r in runs:
@@ -253,3 +256,29 @@ def test_continued_refinement_fp32_krylov_fp64_residual(xamg, prec):
     outer, inner = gpu["iters"], gpu["sub_iters"]
     assert 1 <= outer <= 20
     assert inner <= 1.3 * orc["iters"] + 5, (inner, orc["iters"])
+
+
+def test_checked_preconditioner_runs_its_own_loop(xamg):
+    """A preconditioner with convergence checks (convergence_details check-L2,
+    jacobi.inl:74-89) runs a polling loop of its own inside each outer iteration
+    (pbicgstab.inl:116, 139).  Its convergence must end only its own loop: the outer
+    solve follows the oracle and converges.  With a relative tolerance of 0.135 the
+    preconditioner's sweeps stop at the 4th or 5th of 6, per RHS (masked updates), so
+    its loop ends early every time; the oracle takes 17 outer iterations under every
+    dot summation order.  (A looser inner tolerance such as 0.6 makes the preconditioner
+    vary so much between applications that the oracle alone spans 96-300 iterations.)"""
+    nx, nv = 16, 4
+    row, col, val = xamg.poisson7(nx)
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 12)
+    prec = {"method": "Jacobi", "convergence_details": "check-L2", "max_iters": 6,
+            "rel_tolerance": 0.135, "abs_tolerance": 0.0}
+    params = _params("PBiCGStab", prec, max_iters=300)
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    orc = O.solve(params, O.csr(row, col, val), b, nv)
+    per = [O.solve(params, O.csr(row, col, val), b, nv, nchunks=c) for c in PERTURB_CHUNKS]
+    assert orc["converged"] == [True] * nv
+    _compare(gpu, orc, nv, per)
+    A = poisson7_scipy(nx)
+    R = b.reshape(n, nv) - A @ gpu["x"].reshape(n, nv)
+    assert (np.linalg.norm(R, axis=0) / np.linalg.norm(b.reshape(n, nv), axis=0)).max() <= 1.5e-10

@@ -869,6 +869,18 @@ int xamgx_solver_get_residuals(XAMG_solver_id sid, double *res0, double *res) {
     });
 }
 
+int xamgx_solver_get_rr_info(XAMG_solver_id sid, int *checks, int *stopped) {
+    return guard([&] {
+        Sol *S = slot(g_sols, sid, "solver");
+        SolveStats st;
+        S->top->stats(st, false);
+        if (checks)
+            *checks = st.rr_checks;
+        if (stopped)
+            *stopped = st.rr_stopped;
+    });
+}
+
 int xamgx_set_engine(int batch, int use_graphs, int grid_spmv, int grid_ew, int grid_red,
                      int xcd_remap) {
     return guard([&] {

@@ -63,6 +63,12 @@ struct ConvState {
     int restart;            // FP32 recurrence broke down: restart it from the FP64 residual
     int rstnow;             // (r, r0) lost: restart at the end of this iteration
     double maxres[kMaxNV];  // squared residual at the last FP64 check
+    // residual replacement (convergence_details RR, convergence.h:269-302)
+    double rr_tol;   // next checkpoint of the largest relative recurrence residual
+    int rr_pending;  // set by a check that reached rr_tol: compute the true residual
+    int rr_checks;   // true residuals computed in this solve
+    int rr_stopped;  // a recurrence/true deviation ended this solve
+    int rr_pad;
 };
 
 // Krylov scalars of one BiCGStab/PBiCGStab instance, precision of the iteration.

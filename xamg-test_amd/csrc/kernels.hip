@@ -37,6 +37,8 @@ __device__ void conv_check(const FinArgs<F> &f, const F *rn, int nv, int initial
             cs->res0[l] = (double)rn[l];
             cs->tol[l] = fmax(f.abs_tol, f.rel_tol * sqrt(cs->res0[l]));
         }
+        cs->rr_tol = f.rr_stepping; // convergence.h:184-186
+        cs->rr_pending = cs->rr_checks = cs->rr_stopped = 0;
     }
     for (int l = 0; l < nv; l++)
         if (!cs->status[l])
@@ -62,12 +64,32 @@ __device__ void conv_check(const FinArgs<F> &f, const F *rn, int nv, int initial
             bad = 1;
         }
     }
-    if (flag || bad)
+    if (flag || bad) {
         cs->done = 1;
-    else if (f.iconv != nullptr)
+        return;
+    }
+    if (f.iconv != nullptr)
         for (int l = 0; l < nv; l++)
             if (cs->status[l])
                 f.iconv[l] = (F)0.0;
+    // residual replacement checkpoint (stats.h:151-160, convergence.h:269-284): once the
+    // largest sqrt(res / res0) over every RHS (a converged one at its last value; a NaN
+    // ratio is passed over as by std::max) falls to rr_tol, the next rr_tol is that value
+    // times RR_stepping and the true residual is computed.  At the initial check the
+    // residual is the true one, so only rr_tol moves.
+    if (f.rr_stepping > 0) {
+        double maxrel = 0.0;
+        for (int l = 0; l < nv; l++) {
+            const double ratio = sqrt(cs->res[l] / cs->res0[l]);
+            maxrel = maxrel < ratio ? ratio : maxrel;
+        }
+        if (maxrel <= cs->rr_tol) {
+            cs->rr_tol = maxrel * f.rr_stepping;
+            cs->rr_checks++;
+            if (!initial)
+                cs->rr_pending = 1;
+        }
+    }
 }
 
 // Scalar updates of the Krylov loops, executed by one thread of the last workgroup.
@@ -292,6 +314,22 @@ __device__ void apply_fin(const FinArgs<F> &f, const double *sums, int nv) {
                 cs->restart = 1;
             cs->replace = 0;
             cs->replaced = 1;
+        }
+        break;
+    }
+    case FIN_RR: {
+        // convergence.h:286-302 rr_deviation_check on the true squared norms (in F)
+        int stop = 0;
+        for (int l = 0; l < nv; l++) {
+            const double ratio = sqrt(cs->res[l] / (double)(F)sums[l]);
+            if (ratio > f.rr_deviation || ratio < 1.0 / f.rr_deviation)
+                stop = 1;
+        }
+        cs->rr_pending = 0;
+        if (stop) {
+            cs->rr_stopped = 1;
+            cs->done = 1;
+            cs->halt = 1;
         }
         break;
     }
@@ -877,6 +915,8 @@ __global__ void k_state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hi
         }
         cs->iter = cs->sub_iter = cs->done = cs->halt = cs->nhist = 0;
         cs->hist_cap = hist_cap;
+        cs->rr_tol = 0.0;
+        cs->rr_pending = cs->rr_checks = cs->rr_stopped = 0;
     }
     if (ks != nullptr)
         for (int l = 0; l < kMaxNV; l++)

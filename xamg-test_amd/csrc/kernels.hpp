@@ -84,7 +84,9 @@ enum Fin : int {
     FIN_PCG_ALPHA = 13, // alpha = rho / (p, Ap), masked alpha, -alpha         (pcg.inl:112-128)
     FIN_PCG_BETA = 14,  // beta = (r, z) / rho, rho = (r, z), check on (r, r)  (pcg.inl:130-152)
     FIN_RB_ALPHA = 15,  // delta = (v, r0), alpha = rho / delta, -alpha        (rbicgstab.inl:104-115)
-    FIN_RB_OMEGA = 16   // omega, residual, check, rho, beta, -omega beta      (rbicgstab.inl:136-200)
+    FIN_RB_OMEGA = 16,  // omega, residual, check, rho, beta, -omega beta      (rbicgstab.inl:136-200)
+    FIN_RR = 17         // true residual at a residual-replacement checkpoint: stop the solve
+                        // when a recurrence/true norm ratio leaves [1/dev, dev] (convergence.h:286-302)
 };
 
 template <typename F>
@@ -105,6 +107,9 @@ struct FinArgs {
     // FIN_BETA with the dots (r, r0), (r, r), (r0, r0): ask for a restart of the
     // recurrence when |(r, r0)| < theta ||r|| ||r0|| (bi-orthogonality lost)
     double theta = 0;
+    // residual replacement: RR_stepping and RR_deviation (float32_t in the reference,
+    // convergence.h:61-62, stored here already rounded to float); stepping 0 is off
+    double rr_stepping = 0, rr_deviation = 0;
 };
 
 struct RedArgs {

@@ -61,15 +61,39 @@ hipGraphExec_t capture_batch(int k, const std::function<void()> &body) {
 }
 } // namespace
 
+namespace {
+// Readback slots of the loops that run at one time: a loop's body can run a loop of its
+// own (a preconditioner or smoother with convergence checks), so each nesting depth
+// polls its own pinned flag slots with its own events.
+constexpr int kMaxLoopDepth = 64;
+thread_local int g_loop_depth = 0;
+struct LoopDepth {
+    int d;
+    LoopDepth() : d(g_loop_depth++) {
+        if (d >= kMaxLoopDepth) {
+            g_loop_depth--;
+            throw std::runtime_error("xamg: solver loops nested too deeply");
+        }
+    }
+    ~LoopDepth() { g_loop_depth--; }
+};
+hipEvent_t *loop_events(int depth) {
+    static hipEvent_t ev[kMaxLoopDepth][2] = {};
+    if (!ev[depth][0]) {
+        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[depth][0], hipEventDisableTiming));
+        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[depth][1], hipEventDisableTiming));
+    }
+    return ev[depth];
+}
+} // namespace
+
 int run_loop(int max_iters, ConvState *cs, const std::function<void()> &body) {
     if (g_capturing)
         throw CaptureAbort{};
     Runtime &rt = Runtime::get();
-    static hipEvent_t ev[2] = {nullptr, nullptr};
-    if (!ev[0]) {
-        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-        XAMG_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    }
+    const LoopDepth depth;
+    hipEvent_t *ev = loop_events(depth.d);
+    int *flags = rt.pinned_flags + 2 * depth.d; // 2 * kMaxLoopDepth ints of the 4 KiB block
     int K = std::max(1, engine_cfg().batch);
     // hipGraph replay of full batches.  The graph is captured once per loop, at the
     // second batch: by then every lazy setup (autotuning, dense inverses) has run and a
@@ -95,12 +119,12 @@ int run_loop(int max_iters, ConvState *cs, const std::function<void()> &body) {
         if (cs == nullptr)
             continue;
         int slot = batches & 1;
-        XAMG_HIP_CHECK(hipMemcpyAsync(&rt.pinned_flags[slot], &cs->halt, sizeof(int),
+        XAMG_HIP_CHECK(hipMemcpyAsync(&flags[slot], &cs->halt, sizeof(int),
                                       hipMemcpyDeviceToHost, rt.stream));
         XAMG_HIP_CHECK(hipEventRecord(ev[slot], rt.stream));
         if (pending >= 0) {
             XAMG_HIP_CHECK(hipEventSynchronize(ev[pending]));
-            if (rt.pinned_flags[pending])
+            if (flags[pending])
                 break;
         }
         pending = slot;
@@ -139,6 +163,22 @@ void Solver<T>::read_conf() {
 template <typename T>
 void Solver<T>::setup() {
     read_conf();
+    // stats.h:111-122: RR needs check-L2; its keys come from the residual_control block
+    // (xamg_params_details.h:184-187 defaults).  Both are fixed at setup: the reference
+    // parses convergence_details once (conf_parsed) and reads RR_stepping / RR_deviation
+    // only in convergence::configure (convergence.h:111-124), so renew_params leaves them.
+    rr = 0;
+    if (conv_check && prm.has("convergence_details") &&
+        split_set(prm.get_string("convergence_details")).count("RR")) {
+        rr = 1;
+        if (gp != nullptr && gp->has("residual_control")) {
+            const ParamList &rc = gp->get("residual_control");
+            if (rc.has("RR_stepping"))
+                rr_step = rc.get_float("RR_stepping");
+            if (rc.has("RR_deviation"))
+                rr_dev = rc.get_float("RR_deviation");
+        }
+    }
     state.alloc(sizeof(ConvState) + sizeof(KrylovScalars<T>) + 256);
     cs = state.as<ConvState>();
     ks = (KrylovScalars<T> *)((char *)state.p + ((sizeof(ConvState) + 255) / 256) * 256);
@@ -168,7 +208,33 @@ FinArgs<T> Solver<T>::fin(int f, bool check_mask) {
     a.abs_tol = abs_tol;
     a.rel_tol = rel_tol;
     a.iconv = check_mask ? ks->iconv : nullptr;
+    if (rr && rr_capable) {
+        // float32_t members in the reference (convergence.h:61-62)
+        a.rr_stepping = (double)(float)rr_step;
+        a.rr_deviation = (double)(float)rr_dev;
+    }
     return a;
+}
+
+// True residual r = b - A x and its squared norm at a residual-replacement checkpoint,
+// gated on the flag the check set (stats.h:151-160): a recurrence/true deviation ends
+// the solve through the halt flag.
+// The work vector is buffer `k` of the solver, allocated on first use (residual
+// replacement may also be switched on by renew_params after setup).
+template <typename T>
+static void rr_true_residual(Solver<T> *S, T *x, const T *b, int k, const int *halt) {
+    if (!(S->rr && S->rr_capable))
+        return;
+    while ((int)S->bufs.size() <= k)
+        S->bufs.emplace_back(sizeof(T) * std::max<int64_t>(S->A->n, 1) * S->nv);
+    T *tmp = S->vec(k);
+    SpmvArgs<T> a{};
+    a.x = x;
+    a.y = tmp;
+    a.b = b;
+    RedArgs ra = red_args(halt);
+    ra.gate = &S->cs->rr_pending;
+    spmv<T>(*S->A, S->vprec, EPI_RESID, 1, S->nv, a, ra, S->fin(FIN_RR));
 }
 
 template <typename T>
@@ -179,6 +245,8 @@ void Solver<T>::stats(SolveStats &st, bool with_hist) {
     XAMG_HIP_CHECK(hipMemcpy(&h, cs, sizeof(ConvState), hipMemcpyDeviceToHost));
     st.iters = h.iter;
     st.sub_iters = h.sub_iter;
+    st.rr_checks = h.rr_checks;
+    st.rr_stopped = h.rr_stopped;
     for (int l = 0; l < nv; l++) {
         st.converged[l] = h.status[l];
         st.res0[l] = h.res0[l];
@@ -574,7 +642,9 @@ class Krylov : public Solver<T> {
     std::shared_ptr<MgHierarchy> hier;
 
     void setup() override {
+        this->rr_capable = true;
         Solver<T>::setup();
+        // [r0 r p v s t] [p^ s^], then the true residual of residual replacement
         this->alloc_bufs(pre ? 8 : 6);
         if (pre) {
             XAMG_REQUIRE(this->gp->has("preconditioner"), "PBiCGStab needs a preconditioner block");
@@ -657,6 +727,8 @@ class Krylov : public Solver<T> {
                 a.freeze_zero = 1;
                 ew<T>(EW_AXPBYPCZ, nv, a, H);
             }
+            if (this->rr)
+                rr_true_residual<T>(this, x, b, pre ? 8 : 6, H);
             { // r = s - omega t, rho_n = (r, r0), beta
                 EwArgs<T> a;
                 a.n = A->n;
@@ -699,6 +771,7 @@ class IterRefine : public Solver<double> {
     std::unique_ptr<Solver<double>> inner64;
     DBuf ix32, ib32;
     int sub_iters = 0;
+    int rr_checks = 0, rr_stops = 0; // the inner solves' residual replacement record
 
     void setup() override {
         Solver<double>::setup();
@@ -731,6 +804,8 @@ class IterRefine : public Solver<double> {
         XAMG_HIP_CHECK(hipMemcpyAsync(&h, c, sizeof(ConvState), hipMemcpyDeviceToHost,
                                       Runtime::get().stream));
         Runtime::get().sync();
+        rr_checks += h.rr_checks;
+        rr_stops += h.rr_stopped;
         return h.iter;
     }
     void apply(double *x, bool xzero, const double *b, const int *) override {
@@ -740,7 +815,7 @@ class IterRefine : public Solver<double> {
         if (xzero)
             fill<double>(x, A->n, nv, 0.0);
         resid_norm<double>(this, x, false, b, r, FIN_NORM_INIT, nullptr, false);
-        sub_iters = 0;
+        sub_iters = rr_checks = rr_stops = 0;
         ConvState h;
         XAMG_HIP_CHECK(hipMemcpyAsync(&h, cs, sizeof(ConvState), hipMemcpyDeviceToHost, rt.stream));
         rt.sync();
@@ -779,6 +854,8 @@ class IterRefine : public Solver<double> {
     void stats(SolveStats &st, bool with_hist) override {
         Solver<double>::stats(st, with_hist);
         st.sub_iters = sub_iters;
+        st.rr_checks = rr_checks;
+        st.rr_stopped = rr_stops;
     }
 };
 
@@ -809,15 +886,11 @@ class ContinuedRefine : public Solver<double> {
 
     float *v32(int k) { return b32[k].as<float>(); }
 
-    void setup() override {
-        Solver<double>::setup();
-        XAMG_REQUIRE(gp->has("solver"), "IterativeRefinement needs a solver block");
-        const ParamList &sp = gp->get("solver");
-        const std::string m = sp.get_string("method");
-        XAMG_REQUIRE(m == "BiCGStab" || m == "PBiCGStab" || m == "MergedBiCGStab" ||
-                         m == "MergedPBiCGStab",
-                     "continued refinement runs a BiCGStab-family inner solver");
-        pre = (m == "PBiCGStab" || m == "MergedPBiCGStab");
+    // Keys of the meta_solver list and the inner solver's iteration limit, read at setup
+    // and again by renew (the reference's IR forwards renewed parameters to its inner
+    // solver, solver.h:211-225); the residual history grows with the limit.
+    void read_keys(const GlobalParams &g) {
+        const ParamList &sp = g.get("solver");
         max_inner = sp.has("max_iters") ? (int)sp.get_int("max_iters") : 1000;
         double step = prm.has("replace_step") ? prm.get_float("replace_step") : 1e-2;
         delta2 = step * step;
@@ -827,8 +900,25 @@ class ContinuedRefine : public Solver<double> {
         corr32 = prm.has("correction_fp32") && prm.get_bool("correction_fp32");
         if (!conv_check)
             conv_check = 1; // convergence is always decided on the FP64 residual
-        hist_cap = std::max(2, max_inner + 2);
-        histbuf.alloc(sizeof(double) * hist_cap * nv);
+        const int need = std::max(2, max_inner + 2);
+        if (need > hist_cap || !histbuf.p) {
+            hist_cap = need;
+            histbuf.alloc(sizeof(double) * hist_cap * nv);
+        }
+    }
+
+    void setup() override {
+        Solver<double>::setup();
+        XAMG_REQUIRE(gp->has("solver"), "IterativeRefinement needs a solver block");
+        const ParamList &sp = gp->get("solver");
+        const std::string m = sp.get_string("method");
+        XAMG_REQUIRE(m == "BiCGStab" || m == "PBiCGStab" || m == "MergedBiCGStab" ||
+                         m == "MergedPBiCGStab",
+                     "continued refinement runs a BiCGStab-family inner solver");
+        pre = (m == "PBiCGStab" || m == "MergedPBiCGStab");
+        hist_cap = 0;
+        histbuf.release();
+        read_keys(*gp);
         b32.clear();
         // r0, r, p, v, s, t, the FP32 solution correction, [p^, s^]
         const int nbuf = pre ? 9 : 7;
@@ -850,7 +940,8 @@ class ContinuedRefine : public Solver<double> {
         }
     }
     void renew(const GlobalParams &g) override {
-        Solver<double>::renew(g);
+        Solver<double>::renew(g); // meta_solver keys and tolerances
+        read_keys(g);
         if (precond)
             precond->renew(g);
     }
@@ -1277,6 +1368,7 @@ class PCG : public Solver<T> {
     std::shared_ptr<MgHierarchy> hier;
 
     void setup() override {
+        this->rr_capable = true;
         Solver<T>::setup();
         this->alloc_bufs(4);
         XAMG_REQUIRE(this->gp->has("preconditioner"), "PCG needs a preconditioner block");
@@ -1353,6 +1445,8 @@ class PCG : public Solver<T> {
                 a.w = z;
                 ew<T>(EW_DOTS, 3, nv, a, red_args(H), this->fin(FIN_PCG_BETA, true));
             }
+            if (this->rr)
+                rr_true_residual<T>(this, x, b, 4, H);
             { // p = 1 z + beta p
                 EwArgs<T> a;
                 a.n = A->n;
@@ -1385,6 +1479,7 @@ class RBiCGStab : public Solver<T> {
     std::shared_ptr<MgHierarchy> hier;
 
     void setup() override {
+        this->rr_capable = true;
         Solver<T>::setup();
         this->alloc_bufs(8);
         XAMG_REQUIRE(this->gp->has("preconditioner"), "RBiCGStab needs a preconditioner block");
@@ -1486,6 +1581,8 @@ class RBiCGStab : public Solver<T> {
                 a.freeze_zero = 1;
                 ew<T>(EW_AXPBYPCZ, nv, a, H);
             }
+            if (this->rr)
+                rr_true_residual<T>(this, x, b, 8, H);
             { // 24: z = 1 t_ - omega z (the loop ends here once converged)
                 EwArgs<T> a;
                 a.n = A->n;

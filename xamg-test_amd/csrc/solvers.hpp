@@ -16,6 +16,7 @@ namespace xamg {
 
 struct SolveStats {
     int iters = 0, sub_iters = 0;
+    int rr_checks = 0, rr_stopped = 0; // residual replacement: true residuals, stops
     int converged[kMaxNV] = {0};
     double res0[kMaxNV] = {0}, res[kMaxNV] = {0};
     std::vector<double> hist; // [nhist][nv]
@@ -54,6 +55,12 @@ class Solver : public SolverBase {
     int conv_check = 0;
     double abs_tol = 0, rel_tol = 0;
     int max_iters = 1;
+    // residual replacement (convergence_details RR): on for this solver, whose kind
+    // computes true residuals at the checkpoints (the Krylov solvers: elsewhere the
+    // checked residual is the true one), and the residual_control keys
+    int rr = 0;
+    bool rr_capable = false;
+    double rr_step = 0.1, rr_dev = 2.0;
     int hist_cap = 0;
     DBuf state; // ConvState + KrylovScalars<T>
     DBuf histbuf;

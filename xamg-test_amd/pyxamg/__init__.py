@@ -111,6 +111,7 @@ def lib():
             "xamgx_solver_last_ms": (ci, [sz, vp]),
             "xamgx_solver_get_history": (ci, [sz, vp, ci, vp]),
             "xamgx_solver_get_residuals": (ci, [sz, vp, vp]),
+            "xamgx_solver_get_rr_info": (ci, [sz, vp, vp]),
             "xamgx_set_engine": (ci, [ci, ci, ci, ci, ci, ci]),
             "xamgx_set_spmv_vpt": (ci, [ci, ci, ci]),
             "xamgx_set_spmv_stage": (ci, [ci]),
@@ -518,6 +519,11 @@ class Solver:
         check(lib().xamgx_solver_last_ms(self.id, C.byref(ms)))
         return ms.value
 
+    def rr_info(self):
+        c, s = C.c_int(), C.c_int()
+        check(lib().xamgx_solver_get_rr_info(self.id, C.byref(c), C.byref(s)))
+        return c.value, s.value
+
     def write_record(self, path, x0_norm=None, seconds=0.0, config=""):
         x0 = None if x0_norm is None else np.ascontiguousarray(x0_norm, dtype=np.float64)
         check(lib().xamgx_solver_write_record(self.id, self.x.id, self.b.id,
@@ -640,7 +646,7 @@ def solve_system(row, col, val, b, nv, params, x0=None, eigs=None, keep=False, h
     it, sub = S.iters()
     r0, r = S.residuals()
     out = {"x": x.download(), "iters": it, "sub_iters": sub, "converged": S.converged(),
-           "res0": r0, "res": r, "hist": S.history(), "ms": S.last_ms()}
+           "res0": r0, "res": r, "hist": S.history(), "ms": S.last_ms(), "rr": S.rr_info()}
     if keep:
         out["objs"] = (A, x, bv, P, S)
     else:
