@@ -45,6 +45,15 @@ int xamgx_fill_rhs(double *b, int64_t n, int nv, uint32_t seed);
 int xamgx_spmv_host(XAMG_matrix_id mid, int prec, int mode, int nv, const void *x, void *y);
 /* res[l] = sum_i x[i,l]*y[i,l] on two device vectors (blas.inl:521-589) */
 int xamgx_dot(XAMG_vector_id x, XAMG_vector_id y, int nv, double *res);
+/* The same dot product, allocating nothing: no hipMalloc, hipFree or allocating library
+ * routine.  No counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid it).  Its
+ * per-workgroup partial sums and result live in a fixed device array of 256.1 KiB per
+ * GPU (2048 workgroup slots x 16 RHS), held until the process ends.  Needs XAMG_init
+ * first; rejects a reduction grid above 2048 workgroups (xamgx_set_engine grid_red) and
+ * nv outside {1, 2, 4, 8, 16}.  With the same grid its result is bit-identical to
+ * xamgx_dot's.  Calls on one GPU share the array: calls that overlap in time race. */
+int xamgx_dot_noalloc(XAMG_vector_id x, XAMG_vector_id y, int nv, double *res);
 
 /* ---- measurement ---- */
 /* Average device time (ms) of one y = A x launch over `iters` back-to-back launches

@@ -223,6 +223,36 @@ void do_solve(Sol *S, Vec *x, Vec *b, int nv) {
     print_history(S, st);
 }
 
+// Scratch of xamgx_dot_noalloc: the per-workgroup partial sums of one dot-product launch
+// ([kMaxNV][kNoAllocBlocks] doubles) and the reduced result (kMaxNV doubles).  A
+// file-scope device array, so the entry point allocates nothing; each GPU holds its own
+// copy (256.1 KiB) until the process ends.  Every slot the finish pass reads is written
+// by the same call, so nothing in it has to start at zero.  Calls on one GPU share it:
+// calls that overlap in time (from several host threads) race.
+constexpr int kNoAllocBlocks = 2048;
+constexpr size_t kNoAllocWords = (size_t)kMaxNV * kNoAllocBlocks + kMaxNV;
+static __device__ __align__(256) double scratch[kNoAllocWords];
+
+// Device address of `scratch` per device ordinal.  In host code the array's name is a
+// host-side stand-in; kernels get this address instead.
+constexpr int kMaxDevices = 64;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Address of the current device's copy, resolved once per device (by XAMG_init, or by
+// the first no-allocation call on a device).
+double *noalloc_scratch() {
+    int dev = 0;
+    XAMG_HIP_CHECK(hipGetDevice(&dev));
+    XAMG_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+    if (g_scratch_dev[dev] == nullptr) {
+        void *p = nullptr;
+        XAMG_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        XAMG_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch_dev[dev] = (double *)p;
+    }
+    return g_scratch_dev[dev];
+}
+
 } // namespace
 
 extern "C" {
@@ -231,7 +261,10 @@ int XAMG_init(int argc, char **argv, char *conf) {
     (void)argc;
     (void)argv;
     (void)conf;
-    return guard([&] { Runtime::get(); });
+    return guard([&] {
+        Runtime::get();
+        noalloc_scratch();
+    });
 }
 
 int XAMG_finalize() {
@@ -787,6 +820,51 @@ int xamgx_dot(XAMG_vector_id x, XAMG_vector_id y, int nv, double *res) {
         Runtime::get().sync();
         XAMG_HIP_CHECK(hipMemcpy(res, Runtime::get().ws.result, sizeof(double) * nv,
                                  hipMemcpyDeviceToHost));
+    });
+}
+
+int xamgx_dot_noalloc(XAMG_vector_id x, XAMG_vector_id y, int nv, double *res) {
+    return guard([&] {
+        // Runtime::get() would initialise (and allocate) on a first call
+        XAMG_REQUIRE(Runtime::is_ready(), "xamgx_dot_noalloc: call XAMG_init first");
+        Runtime &rt = Runtime::get();
+        int dev = 0;
+        XAMG_HIP_CHECK(hipGetDevice(&dev));
+        XAMG_REQUIRE(dev == rt.device, "xamgx_dot_noalloc: the current device is not the library's");
+        check_nv(nv);
+        XAMG_REQUIRE(res != nullptr, "xamgx_dot_noalloc: null result pointer");
+        Vec *a = slot(g_vecs, x, "vector"), *b = slot(g_vecs, y, "vector");
+        XAMG_REQUIRE(a->on_device() && b->on_device(), "dot needs device vectors");
+        XAMG_REQUIRE(a->nv == nv && b->nv == nv && a->size == b->size, "dot shape mismatch");
+        // the partial sums of one workgroup per slot: the launch may not use more
+        // workgroups than the scratch holds slots (the reduction order, and so the bits of
+        // the result, follow the workgroup count, as in xamgx_dot)
+        const int grid = launch_cfg().grid_red;
+        if (grid > kNoAllocBlocks) {
+            char m[192];
+            snprintf(m, sizeof(m),
+                     "xamgx_dot_noalloc: reduction grid of %d workgroups exceeds the %d partial "
+                     "slots of its scratch (set grid_red <= %d)",
+                     grid, kNoAllocBlocks, kNoAllocBlocks);
+            throw std::runtime_error(m);
+        }
+        double *ws = noalloc_scratch();
+        RedArgs ra;
+        ra.ws.partials = ws;
+        ra.ws.result = ws + (size_t)kMaxNV * kNoAllocBlocks;
+        ra.ws.counter = nullptr;
+        ra.ws.max_blocks = kNoAllocBlocks;
+        EwArgs<double> e;
+        e.n = a->size;
+        e.z = nullptr; // EW_DOTS stores nothing
+        e.x = a->dev.as<double>();
+        e.w = b->dev.as<double>();
+        // the kernels of xamgx_dot without its copy of x: the same products in the same
+        // per-lane, per-workgroup and finish order
+        launch_ew<double>(EW_DOTS, 1, nv, e, ra, FinArgs<double>{}, rt.stream);
+        XAMG_HIP_CHECK(hipMemcpyAsync(res, ra.ws.result, sizeof(double) * nv, hipMemcpyDeviceToHost,
+                                      rt.stream));
+        XAMG_HIP_CHECK(hipStreamSynchronize(rt.stream));
     });
 }
 

@@ -806,6 +806,7 @@ static void ew_nv(int op, int nd, const EwArgs<T> &a, const RedArgs &r, const Fi
     XAMG_EW_CASE(EW_AXPBY_Z, 0)
     XAMG_EW_CASE(EW_AXPBY_Z, 1)
     XAMG_EW_CASE(EW_AXPBY_Z, 3)
+    XAMG_EW_CASE(EW_DOTS, 1)
     XAMG_EW_CASE(EW_DOTS, 3)
     XAMG_EW_CASE(EW_RB_R, 4)
     XAMG_EW_CASE(EW_AXPBYPCZ, 0)

@@ -7,12 +7,19 @@
 
 namespace xamg {
 
-Runtime &Runtime::get() {
+static Runtime &runtime_instance() {
     static Runtime rt;
+    return rt;
+}
+
+Runtime &Runtime::get() {
+    Runtime &rt = runtime_instance();
     if (!rt.ready)
         rt.init();
     return rt;
 }
+
+bool Runtime::is_ready() { return runtime_instance().ready; }
 
 void Runtime::init() {
     int ndev = 0;

@@ -66,6 +66,8 @@ struct Runtime {
     ReduceWs ws{};
     int *pinned_flags = nullptr; // host-pinned readback slots
     static Runtime &get();
+    // true once init() has run (get() would otherwise initialise, which allocates)
+    static bool is_ready();
     void init();
     void sync() { XAMG_HIP_CHECK(hipStreamSynchronize(stream)); }
     bool ready = false;

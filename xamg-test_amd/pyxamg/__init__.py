@@ -102,6 +102,7 @@ def lib():
             "xamgx_fill_rhs": (ci, [vp, C.c_int64, ci, C.c_uint32]),
             "xamgx_spmv_host": (ci, [sz, ci, ci, ci, vp, vp]),
             "xamgx_dot": (ci, [sz, sz, ci, vp]),
+            "xamgx_dot_noalloc": (ci, [sz, sz, ci, vp]),
             "xamgx_bench_spmv": (ci, [sz, ci, ci, ci, vp]),
             "xamgx_bench_copy": (ci, [C.c_uint64, ci, vp]),
             "xamgx_bench_spmv_epi": (ci, [sz, ci, ci, ci, ci, ci, ci, vp, vp]),
