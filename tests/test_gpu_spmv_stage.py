@@ -1,10 +1,12 @@
 """The LDS-staged SpMV modes (1: column indices and values of each row tile copied to
 LDS first; 2: the same with non-temporal loads of the streamed operands; 3: double-
 buffered staging by direct global-to-LDS loads; 4, 5: modes 2, 3 with 16-bit column
-offsets) against the oracle, bit for bit, at every compiled RHS-per-lane width; tiles
-whose nonzeros exceed the LDS capacity take the direct path inside the same launch; a
-matrix too wide for 16-bit offsets runs modes 4, 5 as 2, 3; a solve gives identical
-iterates in every mode."""
+offsets; 6: each tile's nonzeros spread over the lanes, products summed from LDS in
+column order; 7, 8: modes 2, 4 with 16-byte staging loads and a twice larger tile and
+nonzero chunk) against the oracle, bit for bit, at every compiled RHS-per-lane width;
+tiles whose nonzeros exceed the LDS capacity take the direct path inside the same
+launch; a matrix too wide for 16-bit offsets runs modes 4, 5 as 2, 3; a solve gives
+identical iterates in every mode, on short (7-point) and long (27-point) rows."""
 import numpy as np
 import pytest
 
@@ -16,7 +18,7 @@ pytestmark = pytest.mark.gpu
 VPTS = {1: [1], 2: [1, 2], 4: [2, 4], 8: [2, 4, 8], 16: [4, 8, 16]}
 
 
-@pytest.fixture(params=[1, 2, 3, 4, 5])
+@pytest.fixture(params=[1, 2, 3, 4, 5, 6, 7, 8])
 def staged(xamg, request):
     xamg.set_spmv_stage(request.param)
     yield xamg
@@ -104,7 +106,7 @@ def test_staged_solve_identical_iterates(xamg):
                                  "spectrum_fraction": 0.3}}
     outs = []
     try:
-        for mode in (0, 1, 2, 3, 4, 5):
+        for mode in (0, 1, 2, 3, 4, 5, 6, 7, 8):
             xamg.set_spmv_stage(mode)
             outs.append(xamg.solve_system(row, col, val, b, nv, params, eigs=(1 - c, 1 + c)))
     finally:
@@ -114,6 +116,39 @@ def test_staged_solve_identical_iterates(xamg):
         assert a["iters"] == s["iters"]
         assert np.array_equal(a["x"].view(np.uint64), s["x"].view(np.uint64))
         assert np.array_equal(a["hist"], s["hist"])
+
+
+@pytest.mark.parametrize("prec,nv", [({"method": "Jacobi"}, 4), ({"method": "Jacobi"}, 8)])
+def test_long_rows_solve_identical_iterates_and_oracle(xamg, prec, nv):
+    """27-point rows: every staging mode computes the same SpMV bits and the fused dots
+    are summed in the order of their fixed width in every mode (a forced mode 6, whose
+    order is one RHS per lane, runs them as mode 7), so a solve gives identical iterates
+    whatever mode is forced, and it follows the oracle.  (PBiCGStab + Jacobi: the oracle takes 34 iterations under every dot
+    summation order here; unpreconditioned BiCGStab on this anisotropic operator spans
+    117-126 iterations with the summation order alone.)"""
+    from test_gpu_solvers import PERTURB_CHUNKS, _compare
+    row, col, val = xamg.aniso27(18, 17, 16)
+    n = len(row) - 1
+    b = xamg.rhs(n, nv, 8)
+    params = {"solver": {"method": "PBiCGStab" if prec else "BiCGStab", "rel_tolerance": 1e-10,
+                         "abs_tolerance": 0.0, "max_iters": 2000}}
+    if prec:
+        params["preconditioner"] = dict(prec)
+    outs = []
+    try:
+        for mode in (-1, 0, 2, 3, 6, 7, 8):
+            xamg.set_spmv_stage(mode)
+            outs.append(xamg.solve_system(row, col, val, b, nv, params))
+    finally:
+        xamg.set_spmv_stage(-1)
+    a = outs[0]
+    for s in outs[1:]:
+        assert a["iters"] == s["iters"]
+        assert np.array_equal(a["x"].view(np.uint64), s["x"].view(np.uint64))
+        assert np.array_equal(a["hist"], s["hist"])
+    orc = O.solve(params, O.csr(row, col, val), b, nv)
+    per = [O.solve(params, O.csr(row, col, val), b, nv, nchunks=c) for c in PERTURB_CHUNKS]
+    _compare(a, orc, nv, per)
 
 
 def test_staged_wide_matrix_falls_back(staged):

@@ -117,15 +117,30 @@ struct TuneBufs {
 
 } // namespace
 
-// RHS per lane of the SpMV kernels with fused dots: the default width, narrowed while a
-// tile of rows (256 * VPT / nv of them) would hold more nonzeros on average than the
-// LDS staging capacity (long rows: 27-point stencils).
-int dot_vpt(const DevMatrix &A, int nv) {
+// Matrices whose rows hold 12 or more nonzeros on average (27-point stencils, Galerkin
+// coarse operators) also try staging mode 6, which spreads a tile's nonzeros over the
+// lanes.
+bool long_rows(const DevMatrix &A) { return A.n > 0 && A.nnz >= 12 * A.n; }
+
+// RHS per lane of the SpMV kernels with fused dots (es: vector element bytes): the
+// default width, or on long rows the width of one 16-byte x load per lane (the
+// texture-address unit's cost goes by load instructions), narrowed while a tile of rows
+// (256 * VPT / nv of them) would hold more nonzeros on average than the LDS staging
+// capacity (of modes 7 and 8 on long rows).
+int dot_vpt(const DevMatrix &A, int nv, int es) {
     const std::vector<int> opts = vpt_options(nv);
     const double avg = A.n > 0 ? (double)A.nnz / (double)A.n : 0.0;
+    const bool lr = long_rows(A);
     int v = default_vpt(nv);
+    if (lr) {
+        v = 1;
+        for (int o : opts)
+            if (o * es <= 16)
+                v = std::max(v, o);
+    }
+    const double cap = lr ? kSpmvStageCapVec : kSpmvStageCap;
     for (;;) {
-        if (256.0 * v / nv * avg <= kSpmvStageCap)
+        if (256.0 * v / nv * avg <= cap)
             break;
         int prev = 0;
         for (int o : opts)
@@ -195,15 +210,21 @@ TuneResult run_tuning(DevMatrix &A, Prec vp, int nv, int epi, int nd) {
     // which the dot products are summed, so they follow a rule of the matrix's shape
     // (dot_vpt) and the default workgroup count, and the results do not depend on the
     // tuning; the XCD mapping and the staging mode do not change the order.
-    const std::vector<int> vpts = nd > 0 ? std::vector<int>{dot_vpt(A, nv)} : vpt_options(nv);
+    const int dv = dot_vpt(A, nv, (int)sizeof(T));
+    const std::vector<int> vpts = nd > 0 ? std::vector<int>{dv} : vpt_options(nv);
     const std::vector<int> grids =
         nd > 0 ? std::vector<int>{launch_cfg().grid_red}
                : std::vector<int>{1024, 2048, 4096, 8192, 16384};
-    std::vector<int> modes = {0, 1, 2, 3};
+    // (every mode computes the same SpMV bits; mode 6 sums the dots in the order of one
+    // RHS per lane, so with fused dots it is a candidate only when that is the width)
+    std::vector<int> modes = {0, 1, 2, 3, 7};
     if (A.col16.p != nullptr) {
         modes.push_back(4);
         modes.push_back(5);
+        modes.push_back(8);
     }
+    if (long_rows(A) && (nd == 0 || dv == 1))
+        modes.push_back(6);
     for (int mode : modes)
         for (int remap : {0, 1}) {
             DevMatrix::Tune fin;
@@ -253,12 +274,14 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
         return t;
     t.valid = true; // defaults unless tuned below
     if (nd > 0) // the dot kernels' summation order does not depend on the tuning
-        t.vpt = dot_vpt(*this, nv);
+        t.vpt = dot_vpt(*this, nv, es);
+    if (long_rows(*this))
+        t.mode = 7; // 16-byte staging loads
     // small matrices: launch overheads dominate, keep defaults
     if (!autotune_enabled() || n < 65536)
         return t;
     std::ostringstream k;
-    k << "cold7_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
+    k << "cold9_n" << n << "_c" << ncols << "_nnz" << nnz << "_v" << (int)vp << "_e" << es << "_nv"
       << nv << "_epi" << epi << "_nd" << nd;
     const std::string key = k.str();
     auto &c = cache();

@@ -1025,7 +1025,7 @@ int xamgx_kernel_stats_get(int i, char *name, int len, int64_t *calls, double *m
 
 int xamgx_set_spmv_stage(int mode) {
     return guard([&] {
-        XAMG_REQUIRE(mode >= -1 && mode <= 5, "mode must be -1 (tuned) or 0..5");
+        XAMG_REQUIRE(mode >= -1 && mode <= 8, "mode must be -1 (tuned) or 0..8");
         launch_cfg().spmv_stage = mode;
     });
 }

@@ -771,7 +771,13 @@ int &spmv_vpt_choice(int nv, int es) {
 template <typename V, typename T>
 void launch_spmv(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                  const FinArgs<T> &f, hipStream_t s) {
-    if (a.cfg_mode == 5 && a.col16 != nullptr)
+    if (a.cfg_mode == 6)
+        launch_spmv_nzp<V, T>(epi, ndots, nv, a, r, f, s);
+    else if (a.cfg_mode == 8 && a.col16 != nullptr)
+        launch_spmv_stage8<V, T>(epi, ndots, nv, a, r, f, s);
+    else if (a.cfg_mode == 7 || a.cfg_mode == 8)
+        launch_spmv_stage7<V, T>(epi, ndots, nv, a, r, f, s);
+    else if (a.cfg_mode == 5 && a.col16 != nullptr)
         launch_spmv_stage5<V, T>(epi, ndots, nv, a, r, f, s);
     else if (a.cfg_mode == 4 && a.col16 != nullptr)
         launch_spmv_stage4<V, T>(epi, ndots, nv, a, r, f, s);

@@ -163,7 +163,9 @@ struct SpmvArgs {
     // launch configuration chosen by the autotuner (0 / -1: global defaults); cfg_mode 1
     // stages each row tile's column indices and values through LDS, 2 also loads the
     // streamed operands non-temporally, 3 double-buffers the staging with direct
-    // global-to-LDS loads; 4 and 5 are 2 and 3 with 16-bit column deltas (col16)
+    // global-to-LDS loads; 4 and 5 are 2 and 3 with 16-bit column deltas (col16); 6
+    // spreads each tile's nonzeros over the lanes (long rows); 7 and 8 are 2 and 4 with
+    // 16-byte staging loads, twice the staging capacity and twice the nonzero chunk
     int cfg_vpt = 0, cfg_grid = 0, cfg_remap = -1, cfg_mode = 0;
 };
 
@@ -197,13 +199,14 @@ struct LaunchCfg {
     int grid_ew = 2048;
     int grid_red = 2048;
     int xcd_remap = 1;
-    int spmv_stage = -1; // SpMV staging mode for every launch: -1 as tuned, else 0..5
+    int spmv_stage = -1; // SpMV staging mode for every launch: -1 as tuned, else 0..8
 };
 
 LaunchCfg &launch_cfg();
 
-// Nonzeros of one row tile that SpMV staging modes 1, 2 and 4 hold in LDS.
+// Nonzeros of one row tile that SpMV staging modes 1, 2 and 4 (7 and 8) hold in LDS.
 constexpr int kSpmvStageCap = 2048;
+constexpr int kSpmvStageCapVec = 4096;
 
 // RHS per lane of the SpMV kernel for (nv, element size); 0 selects the default
 // (1 for nv=1, 2 for nv=2, 4 otherwise).  Compiled widths: nv=2 {1,2}, nv=4 {2,4},
@@ -238,6 +241,19 @@ void launch_spmv_stage4(int epi, int ndots, int nv, const SpmvArgs<T> &a, const 
 template <typename V, typename T>
 void launch_spmv_stage5(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
                         const FinArgs<T> &f, hipStream_t s);
+// Staging modes 7 and 8 (spmv_stage7.hip): modes 2 and 4 with 16-byte staging loads, twice
+// the staging capacity and twice the nonzero chunk.
+template <typename V, typename T>
+void launch_spmv_stage7(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                        const FinArgs<T> &f, hipStream_t s);
+template <typename V, typename T>
+void launch_spmv_stage8(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                        const FinArgs<T> &f, hipStream_t s);
+// Staging mode 6 (spmv_nzp.hip): long rows, nonzeros spread over the lanes and the row sums
+// taken from LDS in column order.
+template <typename V, typename T>
+void launch_spmv_nzp(int epi, int ndots, int nv, const SpmvArgs<T> &a, const RedArgs &r,
+                     const FinArgs<T> &f, hipStream_t s);
 
 // Final pass of a fused reduction over G workgroup partials (nd dot products per RHS),
 // applying the solver's scalar update.

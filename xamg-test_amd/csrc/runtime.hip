@@ -60,11 +60,15 @@ static uint16_t f32_to_bf16_bits(float f) {
     return (uint16_t)(u >> 16);
 }
 
+// Column and value arrays carry 16 bytes past their last entry: the vector staging copies
+// of SpMV modes 7 and 8 read whole 16-byte words, the last of which may reach past it.
+constexpr size_t kCsrPad = 16;
+
 void DevMatrix::upload() {
     Runtime &rt = Runtime::get();
     row.alloc(sizeof(uint32_t) * (n + 1));
-    col.alloc(sizeof(uint32_t) * std::max<int64_t>(nnz, 1));
-    val64.alloc(sizeof(double) * std::max<int64_t>(nnz, 1));
+    col.alloc(sizeof(uint32_t) * std::max<int64_t>(nnz, 1) + kCsrPad);
+    val64.alloc(sizeof(double) * std::max<int64_t>(nnz, 1) + kCsrPad);
     XAMG_HIP_CHECK(hipMemcpy(row.p, hrow.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
     if (nnz > 0) {
         XAMG_HIP_CHECK(hipMemcpy(col.p, hcol.data(), sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
@@ -81,8 +85,8 @@ void DevMatrix::upload() {
             }
         }
     if (banded) {
-        // padded by one entry: staging mode 5 copies whole dwords
-        std::vector<int16_t> d16(nnz + 2, 0);
+        // padded: staging mode 5 copies whole dwords, mode 8 whole 16-byte words
+        std::vector<int16_t> d16(nnz + 2 + kCsrPad / 2, 0);
         for (int64_t i = 0; i < n; i++)
             for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
                 d16[j] = (int16_t)((int64_t)hcol[j] - i);
@@ -100,7 +104,7 @@ const void *DevMatrix::values(Prec p) {
             std::vector<float> v(std::max<int64_t>(nnz, 1));
             for (int64_t j = 0; j < nnz; j++)
                 v[j] = (float)hval[j];
-            val32.alloc(sizeof(float) * v.size());
+            val32.alloc(sizeof(float) * v.size() + kCsrPad);
             XAMG_HIP_CHECK(hipMemcpy(val32.p, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
         }
         return val32.p;
@@ -116,7 +120,7 @@ const void *DevMatrix::values(Prec p) {
             for (int64_t i = 0; i < n; i++)
                 for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
                     v[j] = (float)((d[i] * hval[j]) * d[hcol[j]]);
-            val32s.alloc(sizeof(float) * v.size());
+            val32s.alloc(sizeof(float) * v.size() + kCsrPad);
             XAMG_HIP_CHECK(hipMemcpy(val32s.p, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
         }
         return val32s.p;
@@ -125,7 +129,7 @@ const void *DevMatrix::values(Prec p) {
         std::vector<uint16_t> v(std::max<int64_t>(nnz, 1));
         for (int64_t j = 0; j < nnz; j++)
             v[j] = f32_to_bf16_bits((float)hval[j]);
-        valbf.alloc(sizeof(uint16_t) * v.size());
+        valbf.alloc(sizeof(uint16_t) * v.size() + kCsrPad);
         XAMG_HIP_CHECK(hipMemcpy(valbf.p, v.data(), sizeof(uint16_t) * v.size(), hipMemcpyHostToDevice));
     }
     return valbf.p;
@@ -322,12 +326,14 @@ static const char *epi_tag(int epi) {
 
 // Algorithmic HBM bytes of one SpMV launch: row pointers, column indices and values
 // once, x once (ncols rows), every row-wise operand the epilogue reads, every output.
+// Column indices count 4 bytes, the reference's u32 CSR (the 16-bit offsets of staging
+// modes 4 and 5 are this build's compression, not a smaller algorithm).
 template <typename T>
 static double spmv_launch_bytes(const DevMatrix &A, Prec vp, int epi, int nd, int nv,
                                 const SpmvArgs<T> &a) {
     const double sv = vp == Prec::F64 ? 8 : vp == Prec::BF16 ? 2 : 4, st = sizeof(T);
     const double E = (double)A.n * nv * st, Dv = (double)A.n * st;
-    const double si = a.cfg_mode >= 4 ? 2.0 : 4.0; // column index bytes
+    const double si = 4.0; // column index bytes
     double b = 4.0 * (A.n + 1) + A.nnz * (si + sv) + (double)A.ncols * nv * st;
     if (epi == EPI_RESID_MX)
         b += (double)A.n * nv * 4 * (1 + (a.yo2 != nullptr) + (a.yo3 != nullptr));
@@ -397,10 +403,17 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
         a.cfg_remap = t.remap;
         a.cfg_mode = t.mode;
     }
-    if (launch_cfg().spmv_stage >= 0)
-        a.cfg_mode = launch_cfg().spmv_stage;
-    if (a.cfg_mode >= 4 && a.col16 == nullptr)
+    if (launch_cfg().spmv_stage >= 0) {
+        // a forced staging mode applies where it cannot change the bits: mode 6 sums the
+        // fused dots in the order of one RHS per lane, so a launch with dots at another
+        // width takes mode 7 instead
+        const int m = launch_cfg().spmv_stage;
+        a.cfg_mode = (m == 6 && nd > 0 && dot_vpt(A, nv, (int)sizeof(T)) != 1) ? 7 : m;
+    }
+    if ((a.cfg_mode == 4 || a.cfg_mode == 5) && a.col16 == nullptr)
         a.cfg_mode -= 2; // 16-bit column deltas need a banded matrix
+    if (a.cfg_mode == 8 && a.col16 == nullptr)
+        a.cfg_mode = 7;
     KernelTimer &kt = KernelTimer::get();
     int tok = -1;
     if (kt.on()) {

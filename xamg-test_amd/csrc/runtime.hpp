@@ -168,7 +168,9 @@ struct CacheFlush {
 
 // RHS per lane of the SpMV kernels with fused dots on this matrix (a rule of its shape,
 // so the dots' summation order never depends on timing).
-int dot_vpt(const DevMatrix &A, int nv);
+int dot_vpt(const DevMatrix &A, int nv, int es);
+// Rows of 12 or more nonzeros on average: staging mode 6 is a tuning candidate.
+bool long_rows(const DevMatrix &A);
 
 // Mean device time (ms) of one SpMV launch with the launch configuration in `a`: reps
 // back-to-back launches, or with `flush` each launch alone after a cache flush.

@@ -155,112 +155,83 @@ __device__ __forceinline__ void st_vec(T *__restrict__ p, const T (&v)[VPT]) {
 // (col16, kept for matrices whose entries all lie within 32767 of the diagonal): half
 // the index bytes.
 // ---------------------------------------------------------------------------
-// One row of the SpMV family: the lanes of row i (VPT right-hand sides each, from l0)
-// walk nonzeros j0..j1 in chunks of U (all column indices, values and x segments of a
-// chunk loaded first, then added in ascending column order, csr_matvec.inl:213-231),
-// apply the epilogue, store, and add the row's terms to the fused dots.  nz(q, c, v)
-// fetches column index and value of entry q (from global memory or an LDS copy).
-template <typename V, typename T, int NV, int VPT, int EPI, int ND, bool NT, typename NZ>
-__device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0, uint32_t j0,
-                                         uint32_t j1, NZ &&nz,
-                                         double (&dacc)[ND > 0 ? ND : 1][VPT]) {
-    constexpr int U = (16 / VPT) > 2 ? (16 / VPT) : 2;
-    const int64_t e = i * NV + l0;
-    T acc[VPT];
-    if constexpr (EPI == EPI_AXPY) {
-        ld_vec<T, VPT, NT>(a.y + e, acc);
-    } else {
-#pragma unroll
-        for (int v = 0; v < VPT; v++)
-            acc[v] = (T)0;
-    }
-    // the epilogue's own row operands are loaded before the gathers so their
-    // latency overlaps the nonzero loop.  They are read once per launch and not again
-    // before other kernels have streamed more than the last-level cache holds, so they
-    // are loaded non-temporally: they do not displace the matrix and the vector the next
-    // kernel gathers from the 256 MB last-level cache (a first Chebyshev step's b is its
-    // gathered x and keeps ordinary loads).
+// Row-wise operands of an epilogue (b, the previous iterate, the second dot operand, the
+// row scaling), loaded before the row's gathers so their latency overlaps them.  They are
+// read once per launch and not again before other kernels have streamed more than the
+// last-level cache holds, so they are loaded non-temporally: they do not displace the
+// matrix and the vector the next kernel gathers from the 256 MB last-level cache (a first
+// Chebyshev step's b is its gathered x and keeps ordinary loads).
+template <typename T, int VPT>
+struct RowOps {
+    T pb[VPT], pw[VPT], px[VPT];
+    T pdi;
+};
+
+template <typename T, int NV, int VPT, int EPI, int ND>
+__device__ __forceinline__ void row_ops_load(const SpmvArgs<T> &a, int64_t i, int64_t e,
+                                             RowOps<T, VPT> &o) {
     constexpr bool kNeedB = EPI != EPI_AXY && EPI != EPI_AXPY;
     constexpr bool NTB = EPI != EPI_CHEB_SF;
     constexpr bool kNeedD = EPI == EPI_JACOBI || epi_cheb(EPI);
-    T pb[VPT], pw[VPT], px[VPT];
-    T pdi = (T)0;
+    o.pdi = (T)0;
     if constexpr (kNeedB)
-        ld_vec<T, VPT, NTB>(a.b + e, pb);
+        ld_vec<T, VPT, NTB>(a.b + e, o.pb);
     if constexpr (kNeedD)
-        pdi = a.d[i];
+        o.pdi = a.d[i];
     if constexpr (EPI == EPI_JACOBI) {
-        ld_vec<T, VPT, true>(a.xin + e, px);
+        ld_vec<T, VPT, true>(a.xin + e, o.px);
     } else if constexpr (epi_cheb_last(EPI)) {
         if (a.xin != nullptr)
-            ld_vec<T, VPT, true>(a.xin + e, px);
+            ld_vec<T, VPT, true>(a.xin + e, o.px);
     }
     if constexpr (ND == 3) {
-        ld_vec<T, VPT, true>(a.w + e, pw);
+        ld_vec<T, VPT, true>(a.w + e, o.pw);
     } else if constexpr (ND == 1) {
         if (a.w != nullptr)
-            ld_vec<T, VPT, true>(a.w + e, pw);
+            ld_vec<T, VPT, true>(a.w + e, o.pw);
     }
-    for (uint32_t j = j0; j < j1; j += U) {
-        uint32_t c[U];
-        T av[U];
-        T xv[U][VPT];
+}
+
+// The gathered x segment of one nonzero as the row sum uses it: the Chebyshev epilogues
+// scale it by the column's diagonal factor first (chebyshev.inl:169-175), in the
+// reference's operation order.
+template <typename T, int VPT, int EPI>
+__device__ __forceinline__ void x_transform(const SpmvArgs<T> &a, uint32_t c, T (&xv)[VPT]) {
+    if constexpr (EPI == EPI_CHEB_SF) {
+        // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
+        const T dc = a.dcol[c];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const bool ok = j + u < j1;
-            T v;
-            nz(ok ? j + u : j0, c[u], v);
-            av[u] = ok ? v : (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            ld_vec<T, VPT>(a.x + (int64_t)c[u] * NV + l0, xv[u]);
-        if constexpr (EPI == EPI_CHEB_SF) {
+        for (int v = 0; v < VPT; v++)
+            xv[v] = a.xcoef * (xv[v] * dc);
+    } else if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
+        if (a.x_is_rhs && a.prescaled) {
             // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
+            const T dc = a.dcol[c];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const T dc = a.dcol[c[u]];
+            for (int v = 0; v < VPT; v++)
+                xv[v] = a.xcoef * (xv[v] * dc);
+        } else if (a.x_is_rhs) {
+            // first Horner step read straight from the right-hand side:
+            // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
+            const T dc = a.dcol[c];
 #pragma unroll
-                for (int v = 0; v < VPT; v++)
-                    xv[u][v] = a.xcoef * (xv[u][v] * dc);
-            }
-        } else if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
-            if (a.x_is_rhs && a.prescaled) {
-                // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
+            for (int v = 0; v < VPT; v++)
+                xv[v] = (a.xcoef * (xv[v] * dc)) * dc;
+        } else if (!a.prescaled) {
+            const T dc = a.dcol[c];
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const T dc = a.dcol[c[u]];
-#pragma unroll
-                    for (int v = 0; v < VPT; v++)
-                        xv[u][v] = a.xcoef * (xv[u][v] * dc);
-                }
-            } else if (a.x_is_rhs) {
-                // first Horner step read straight from the right-hand side:
-                // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const T dc = a.dcol[c[u]];
-#pragma unroll
-                    for (int v = 0; v < VPT; v++)
-                        xv[u][v] = (a.xcoef * (xv[u][v] * dc)) * dc;
-                }
-            } else if (!a.prescaled) {
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const T dc = a.dcol[c[u]];
-#pragma unroll
-                    for (int v = 0; v < VPT; v++)
-                        xv[u][v] = xv[u][v] * dc;
-                }
-            }
+            for (int v = 0; v < VPT; v++)
+                xv[v] = xv[v] * dc;
         }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (j + u < j1)
-#pragma unroll
-                for (int v = 0; v < VPT; v++)
-                    acc[v] = acc[v] + xv[u][v] * av[u];
     }
+}
+
+// Epilogue of one row's VPT right-hand sides from e = i * NV + l0 on: the row sums acc,
+// the output stores and the row's terms of the fused dots.
+template <typename T, int NV, int VPT, int EPI, int ND>
+__device__ __forceinline__ void row_finish(const SpmvArgs<T> &a, int64_t e, int l0,
+                                           const T (&acc)[VPT], const RowOps<T, VPT> &o,
+                                           double (&dacc)[ND > 0 ? ND : 1][VPT]) {
     T out[VPT];
     if constexpr (EPI == EPI_AXY || EPI == EPI_AXPY) {
 #pragma unroll
@@ -269,17 +240,17 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
     } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
 #pragma unroll
         for (int v = 0; v < VPT; v++)
-            out[v] = (T)1.0 * pb[v] + (T)(-1.0) * acc[v];
+            out[v] = (T)1.0 * o.pb[v] + (T)(-1.0) * acc[v];
     } else if constexpr (EPI == EPI_JACOBI) {
-        const T di = pdi;
+        const T di = o.pdi;
 #pragma unroll
         for (int v = 0; v < VPT; v++)
-            out[v] = px[v] + (a.wcoef[l0 + v] * di) * (pb[v] - acc[v]);
+            out[v] = o.px[v] + (a.wcoef[l0 + v] * di) * (o.pb[v] - acc[v]);
     } else {
-        const T di = pdi;
+        const T di = o.pdi;
 #pragma unroll
         for (int v = 0; v < VPT; v++) {
-            const T rr = pb[v] * di;
+            const T rr = o.pb[v] * di;
             T t2 = acc[v];
             if constexpr (!epi_cheb_scaled(EPI))
                 t2 = a.prescaled ? acc[v] : acc[v] * di;
@@ -292,7 +263,7 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
             if (a.xin != nullptr) {
 #pragma unroll
                 for (int v = 0; v < VPT; v++)
-                    out[v] = (T)1.0 * out[v] + (T)1.0 * px[v];
+                    out[v] = (T)1.0 * out[v] + (T)1.0 * o.px[v];
             }
         }
     }
@@ -317,7 +288,7 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
         if (a.w != nullptr) {
 #pragma unroll
             for (int v = 0; v < VPT; v++)
-                dacc[0][v] += (double)out[v] * (double)pw[v];
+                dacc[0][v] += (double)out[v] * (double)o.pw[v];
         } else {
 #pragma unroll
             for (int v = 0; v < VPT; v++)
@@ -326,15 +297,65 @@ __device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0
     } else if constexpr (ND == 3) {
 #pragma unroll
         for (int v = 0; v < VPT; v++) {
-            const double o = (double)out[v], wv = (double)pw[v];
-            dacc[0][v] += o * wv;
-            dacc[1][v] += o * o;
+            const double ov = (double)out[v], wv = (double)o.pw[v];
+            dacc[0][v] += ov * wv;
+            dacc[1][v] += ov * ov;
             dacc[2][v] += wv * wv;
         }
     }
 }
 
+// One row of the SpMV family: the lanes of row i (VPT right-hand sides each, from l0)
+// walk nonzeros j0..j1 in chunks of U (all column indices, values and x segments of a
+// chunk loaded first, then added in ascending column order, csr_matvec.inl:213-231),
+// apply the epilogue, store, and add the row's terms to the fused dots.  nz(q, c, v)
+// fetches column index and value of entry q (from global memory or an LDS copy).
+template <typename V, typename T, int NV, int VPT, int EPI, int ND, bool NT, typename NZ,
+          int UF = 1>
+__device__ __forceinline__ void spmv_row(const SpmvArgs<T> &a, int64_t i, int l0, uint32_t j0,
+                                         uint32_t j1, NZ &&nz,
+                                         double (&dacc)[ND > 0 ? ND : 1][VPT]) {
+    constexpr int U = ((16 / VPT) > 2 ? (16 / VPT) : 2) * UF;
+    const int64_t e = i * NV + l0;
+    T acc[VPT];
+    if constexpr (EPI == EPI_AXPY) {
+        ld_vec<T, VPT, NT>(a.y + e, acc);
+    } else {
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            acc[v] = (T)0;
+    }
+    RowOps<T, VPT> o;
+    row_ops_load<T, NV, VPT, EPI, ND>(a, i, e, o);
+    for (uint32_t j = j0; j < j1; j += U) {
+        uint32_t c[U];
+        T av[U];
+        T xv[U][VPT];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool ok = j + u < j1;
+            T v;
+            nz(ok ? j + u : j0, c[u], v);
+            av[u] = ok ? v : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            ld_vec<T, VPT>(a.x + (int64_t)c[u] * NV + l0, xv[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            x_transform<T, VPT, EPI>(a, c[u], xv[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (j + u < j1)
+#pragma unroll
+                for (int v = 0; v < VPT; v++)
+                    acc[v] = acc[v] + xv[u][v] * av[u];
+    }
+    row_finish<T, NV, VPT, EPI, ND>(a, e, l0, acc, o, dacc);
+}
+
 constexpr int kStageCap = kSpmvStageCap;
+constexpr int kStageCapVec = kSpmvStageCapVec;
 
 template <typename V, typename T, int NV, int VPT, int EPI, int ND, int STAGE>
 __global__ void __launch_bounds__(kBlock)
@@ -359,24 +380,47 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int v = 0; v < VPT; v++)
             dacc[k][v] = 0.0;
-    constexpr int CAP = STAGE ? kStageCap : 1;
-    constexpr bool NT = STAGE == 2 || STAGE == 4;
-    constexpr bool D16 = STAGE == 4; // 16-bit column deltas
-    __shared__ uint32_t s_col[D16 ? 1 : CAP];
-    __shared__ int16_t s_c16[D16 ? CAP : 1];
-    __shared__ V s_val[CAP];
+    // STAGE 7 / 8: as 2 / 4 with the tile copied by 16-byte loads per lane (a
+    // wavefront load moves 1 KB instead of 128-256 bytes: the texture-address unit's cost
+    // goes by instructions), twice the staging capacity and twice the nonzero chunk
+    // (rows of 27 nonzeros and more)
+    constexpr bool VEC = STAGE >= 7;
+    constexpr int CAP = STAGE ? (VEC ? kStageCapVec : kStageCap) : 1;
+    constexpr bool NT = STAGE == 2 || STAGE == 4 || VEC;
+    constexpr bool D16 = STAGE == 4 || STAGE == 8; // 16-bit column deltas
+    // entries per 16-byte word of the column and value arrays; a vector copy starts at
+    // the word holding the tile's first entry and ends with the word holding its last
+    constexpr int EC = D16 ? 8 : 4, EV = (int)(16 / sizeof(V));
+    constexpr int SLACK = VEC ? 16 : 0;
+    __shared__ __align__(16) uint32_t s_col[D16 ? 4 : CAP + SLACK];
+    __shared__ __align__(16) int16_t s_c16[D16 ? CAP + SLACK : 8];
+    __shared__ __align__(16) V s_val[CAP + SLACK];
     const int16_t *__restrict__ col16 = a.col16;
+    constexpr int UF = VEC ? 2 : 1;
     for (int64_t base = r0; base < r1; base += RPB) {
         const int64_t i = base + lr;
-        uint32_t jb = 0;
+        uint32_t jc = 0, jv = 0; // entries at LDS index 0 of the column and value copies
         bool staged = false;
         if constexpr (STAGE) {
             const int64_t bend = base + RPB < r1 ? base + RPB : r1;
             const uint32_t jA = row[base], jB = row[bend];
             staged = jB - jA <= (uint32_t)CAP;
-            jb = jA;
+            jc = jv = jA;
             __syncthreads(); // the previous tile's rows are done with the LDS copy
-            if (staged)
+            if constexpr (VEC) {
+                jc = jA & ~(uint32_t)(EC - 1);
+                jv = jA & ~(uint32_t)(EV - 1);
+                if (staged) {
+                    const uint32_t nwc = (jB - jc + EC - 1) / EC, nwv = (jB - jv + EV - 1) / EV;
+                    const u32x4_t *gc = D16 ? (const u32x4_t *)(col16 + jc) : (const u32x4_t *)(col + jc);
+                    u32x4_t *sc = D16 ? (u32x4_t *)s_c16 : (u32x4_t *)s_col;
+                    for (uint32_t k = tid; k < nwc; k += kBlock)
+                        sc[k] = __builtin_nontemporal_load(gc + k);
+                    const u32x4_t *gv = (const u32x4_t *)(val + jv);
+                    for (uint32_t k = tid; k < nwv; k += kBlock)
+                        ((u32x4_t *)s_val)[k] = __builtin_nontemporal_load(gv + k);
+                }
+            } else if (staged) {
                 for (uint32_t k = tid; k < jB - jA; k += kBlock) {
                     if constexpr (D16)
                         s_c16[k] = ld_nt(col16 + jA + k);
@@ -384,18 +428,18 @@ __global__ void __launch_bounds__(kBlock)
                         s_col[k] = NT ? ld_nt(col + jA + k) : col[jA + k];
                     s_val[k] = NT ? ld_nt(val + jA + k) : val[jA + k];
                 }
+            }
             __syncthreads();
         }
         if (i < r1) {
             const uint32_t j0 = row[i], j1 = row[i + 1];
-            spmv_row<V, T, NV, VPT, EPI, ND, NT>(a, i, l0, j0, j1,
-                                                 [&](uint32_t q, uint32_t &c, T &v) {
+            auto nzf = [&](uint32_t q, uint32_t &c, T &v) {
                                                      if (STAGE && staged) {
                                                          if constexpr (D16)
-                                                             c = (uint32_t)(i + s_c16[q - jb]);
+                                                             c = (uint32_t)(i + s_c16[q - jc]);
                                                          else
-                                                             c = s_col[q - jb];
-                                                         v = to_t<T>(s_val[q - jb]);
+                                                             c = s_col[q - jc];
+                                                         v = to_t<T>(s_val[q - jv]);
                                                      } else {
                                                          // a tile too large for LDS: plain loads
                                                          // (non-temporal loads of entries a row
@@ -406,8 +450,8 @@ __global__ void __launch_bounds__(kBlock)
                                                              c = col[q];
                                                          v = to_t<T>(val[q]);
                                                      }
-                                                 },
-                                                 dacc);
+                                                 };
+            spmv_row<V, T, NV, VPT, EPI, ND, NT, decltype(nzf) &, UF>(a, i, l0, j0, j1, nzf, dacc);
         }
     }
     if constexpr (ND > 0)
