@@ -60,8 +60,11 @@ int xamgx_dot_noalloc(XAMG_vector_id x, XAMG_vector_id y, int nv, double *res);
  * after 10 warm-ups, timed with HIP events on the launch stream. */
 int xamgx_bench_spmv(XAMG_matrix_id mid, int prec, int nv, int iters, double *avg_ms);
 /* Sustained copy bandwidth (GB/s, read+write bytes) of a streaming copy kernel over
- * `bytes` bytes, best of `iters` launches. */
+ * `bytes` bytes, best of `iters` launches (each timed from the kernel's start to its end). */
 int xamgx_bench_copy(uint64_t bytes, int iters, double *gbs);
+/* STREAM triad a = b + s c on FP64 arrays of `bytes` bytes each: GB/s of the 3 x bytes
+ * moved, best of `iters` launches. */
+int xamgx_bench_triad(uint64_t bytes, int iters, double *gbs);
 /* Average device time (ms) of one launch of an SpMV epilogue (0 y=Ax, 1 y+=Ax, 2 residual,
  * 3 Jacobi step, 4 Chebyshev step, 5 last Chebyshev step, 6 FP64 residual stored as FP32,
  * 7 and 8 Chebyshev step and last step on the scaled operator after the first step, 9 first

@@ -1060,29 +1060,51 @@ int xamgx_matrix_get_eigs(XAMG_matrix_id mid, double *lo, double *hi) {
 
 } // extern "C"
 
+// Best rate of `iters` launches of a streaming kernel, each timed by HIP events its
+// dispatch stamps at the kernel's start and end.
+template <typename L>
+static double best_rate(int iters, double bytes_moved, L &&launch) {
+    Runtime &rt = Runtime::get();
+    hipEvent_t e0, e1;
+    XAMG_HIP_CHECK(hipEventCreate(&e0));
+    XAMG_HIP_CHECK(hipEventCreate(&e1));
+    double best = 0;
+    launch(); // warm-up
+    for (int i = 0; i < iters; i++) {
+        launch_events() = LaunchEvents{e0, e1};
+        launch();
+        launch_events() = LaunchEvents{};
+        XAMG_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        best = std::max(best, bytes_moved / (ms * 1e-3) / 1e9);
+    }
+    (void)rt;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
 extern "C" int xamgx_bench_copy(uint64_t bytes, int iters, double *gbs) {
     return guard([&] {
         Runtime &rt = Runtime::get();
         bytes = (bytes / 16) * 16;
         DBuf a(bytes), b(bytes);
         XAMG_HIP_CHECK(hipMemset(a.p, 1, bytes));
-        hipEvent_t e0, e1;
-        XAMG_HIP_CHECK(hipEventCreate(&e0));
-        XAMG_HIP_CHECK(hipEventCreate(&e1));
-        double best = 0;
-        launch_copy(a.p, b.p, (int64_t)bytes, rt.stream);
-        for (int i = 0; i < iters; i++) {
-            XAMG_HIP_CHECK(hipEventRecord(e0, rt.stream));
-            launch_copy(a.p, b.p, (int64_t)bytes, rt.stream);
-            XAMG_HIP_CHECK(hipEventRecord(e1, rt.stream));
-            XAMG_HIP_CHECK(hipEventSynchronize(e1));
-            float ms = 0;
-            XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            best = std::max(best, 2.0 * (double)bytes / (ms * 1e-3) / 1e9);
-        }
-        *gbs = best;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        *gbs = best_rate(iters, 2.0 * (double)bytes,
+                         [&] { launch_copy(a.p, b.p, (int64_t)bytes, rt.stream); });
+    });
+}
+
+extern "C" int xamgx_bench_triad(uint64_t bytes, int iters, double *gbs) {
+    return guard([&] {
+        Runtime &rt = Runtime::get();
+        bytes = (bytes / 16) * 16;
+        DBuf a(bytes), b(bytes), c(bytes);
+        fill<double>(b.as<double>(), (int64_t)(bytes / 8), 1, 1.0);
+        fill<double>(c.as<double>(), (int64_t)(bytes / 8), 1, 2.0);
+        *gbs = best_rate(iters, 3.0 * (double)bytes,
+                         [&] { launch_triad(a.p, b.p, c.p, 3.0, (int64_t)bytes, rt.stream); });
     });
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -88,4 +89,26 @@ struct ReduceWs {
     int max_blocks;
 };
 
+// Events of the library's per-launch timer: while `stop` is set, the launches below carry
+// HIP events that their dispatches stamp at the kernel's start (the first launch) and
+// end (every launch re-stamps `stop`, so it ends up at the last kernel's end), so a class
+// is timed from its first kernel's start to its last kernel's end, without the gaps
+// between dispatches that events recorded on the stream around them would include.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+LaunchEvents &launch_events();
+
 } // namespace xamg
+
+// hipLaunchKernelGGL, or with the timer's events when it is armed
+#define XAMG_LAUNCH(K, G, B, SH, S, ...)                                                           \
+    do {                                                                                           \
+        ::xamg::LaunchEvents &le_ = ::xamg::launch_events();                                       \
+        if (le_.stop != nullptr) {                                                                 \
+            hipExtLaunchKernelGGL(K, G, B, SH, S, le_.start, le_.stop, 0, __VA_ARGS__);            \
+            le_.start = nullptr;                                                                   \
+        } else {                                                                                   \
+            hipLaunchKernelGGL(K, G, B, SH, S, __VA_ARGS__);                                       \
+        }                                                                                          \
+    } while (0)

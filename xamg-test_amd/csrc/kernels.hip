@@ -16,6 +16,11 @@ LaunchCfg &launch_cfg() {
     return cfg;
 }
 
+LaunchEvents &launch_events() {
+    thread_local LaunchEvents ev;
+    return ev;
+}
+
 // Masked coefficient of a per-RHS solution update (pbicgstab.inl:163-165 forms it as
 // mask * coefficient).  A converged RHS gets exactly 0 even when its own recurrence has
 // since broken down (0/0 once its residual underflows), where the product would give
@@ -502,11 +507,11 @@ __global__ void __launch_bounds__(kFinBlock) k_finish(RedArgs r, FinArgs<T> f, i
 template <typename T, int NV>
 void launch_finish(int nd, const RedArgs &r, const FinArgs<T> &f, int G, hipStream_t s) {
     if (nd == 1)
-        hipLaunchKernelGGL((k_finish<T, NV, 1>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
+        XAMG_LAUNCH((k_finish<T, NV, 1>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
     else if (nd == 3)
-        hipLaunchKernelGGL((k_finish<T, NV, 3>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
+        XAMG_LAUNCH((k_finish<T, NV, 3>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
     else if (nd == 4)
-        hipLaunchKernelGGL((k_finish<T, NV, 4>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
+        XAMG_LAUNCH((k_finish<T, NV, 4>), dim3(1), dim3(kFinBlock), 0, s, r, f, G);
 }
 
 // ---------------------------------------------------------------------------
@@ -804,7 +809,7 @@ static void ew_nv(int op, int nd, const EwArgs<T> &a, const RedArgs &r, const Fi
     dim3 g(G), b(kBlock);
 #define XAMG_EW_CASE(O, D)                                                                         \
     if (op == O && nd == D) {                                                                      \
-        hipLaunchKernelGGL((k_ew<T, NV, O, D>), g, b, 0, s, a, r, f, c.xcd_remap);                 \
+        XAMG_LAUNCH((k_ew<T, NV, O, D>), g, b, 0, s, a, r, f, c.xcd_remap);                        \
         launch_finish<T, NV>(D, r, f, G, s);                                                       \
         return;                                                                                    \
     }
@@ -1005,8 +1010,42 @@ __global__ void __launch_bounds__(kBlock) k_copy(const uint4 *__restrict__ src, 
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s) {
     const int64_t n16 = bytes / 16;
     int64_t grid = (n16 + 4 * kBlock - 1) / (4 * kBlock);
-    hipLaunchKernelGGL(k_copy, dim3(grid > 0 ? grid : 1), dim3(kBlock), 0, s, (const uint4 *)src,
-                       (uint4 *)dst, n16);
+    XAMG_LAUNCH(k_copy, dim3(grid > 0 ? grid : 1), dim3(kBlock), 0, s, (const uint4 *)src,
+                (uint4 *)dst, n16);
+    XAMG_HIP_CHECK(hipGetLastError());
+}
+
+// STREAM triad a = b + s c on FP64 arrays (the second bandwidth reference): each thread
+// handles 4 x 16 bytes of each array, both loads of all four issued before the stores.
+__global__ void __launch_bounds__(kBlock) k_triad(double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                  const double2 *__restrict__ c, double sc, int64_t n2) {
+    const int64_t i0 = ((int64_t)blockIdx.x * kBlock) * 4 + threadIdx.x;
+    double2 vb[4], vc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int64_t i = i0 + (int64_t)q * kBlock;
+        if (i < n2) {
+            vb[q] = b[i];
+            vc[q] = c[i];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int64_t i = i0 + (int64_t)q * kBlock;
+        if (i < n2) {
+            double2 r;
+            r.x = vb[q].x + sc * vc[q].x;
+            r.y = vb[q].y + sc * vc[q].y;
+            a[i] = r;
+        }
+    }
+}
+
+void launch_triad(void *a, const void *b, const void *c, double sc, int64_t bytes, hipStream_t s) {
+    const int64_t n2 = bytes / 16;
+    int64_t grid = (n2 + 4 * kBlock - 1) / (4 * kBlock);
+    XAMG_LAUNCH(k_triad, dim3(grid > 0 ? grid : 1), dim3(kBlock), 0, s, (double2 *)a,
+                (const double2 *)b, (const double2 *)c, sc, n2);
     XAMG_HIP_CHECK(hipGetLastError());
 }
 

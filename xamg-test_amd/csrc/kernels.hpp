@@ -281,4 +281,6 @@ void launch_state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap
 
 namespace xamg {
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
+// a = b + sc * c over `bytes` bytes of FP64 per array
+void launch_triad(void *a, const void *b, const void *c, double sc, int64_t bytes, hipStream_t s);
 }

@@ -280,7 +280,10 @@ int KernelTimer::begin(const std::string &cls, double bytes) {
         k = it->second;
     }
     Pending p{k, bytes, take(), take()};
-    XAMG_HIP_CHECK(hipEventRecord(p.e0, s));
+    // the class's launches stamp the events at their kernels' start and end
+    LaunchEvents &le = launch_events();
+    le.start = p.e0;
+    le.stop = p.e1;
     pend_.push_back(p);
     return (int)pend_.size() - 1;
 }
@@ -288,7 +291,18 @@ int KernelTimer::begin(const std::string &cls, double bytes) {
 void KernelTimer::end(int token) {
     if (token < 0)
         return;
-    XAMG_HIP_CHECK(hipEventRecord(pend_[token].e1, Runtime::get().stream));
+    launch_events() = LaunchEvents{};
+}
+
+void KernelTimer::cancel(int token) {
+    if (token < 0)
+        return;
+    launch_events() = LaunchEvents{};
+    if (token == (int)pend_.size() - 1) {
+        pool_.push_back(pend_.back().e0);
+        pool_.push_back(pend_.back().e1);
+        pend_.pop_back();
+    }
 }
 
 const std::vector<KernelTimer::Stat> &KernelTimer::stats() {
@@ -425,13 +439,18 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
                  (long long)A.n);
         tok = kt.begin(cls, spmv_launch_bytes<T>(A, vp, epi, nd, nv, a));
     }
-    A.online_begin(slot, trial);
-    if constexpr (std::is_same<T, double>::value) {
-        launch_spmv<double, double>(epi, nd, nv, a, r, f, s);
-    } else if (vp == Prec::BF16) {
-        launch_spmv<__hip_bfloat16, float>(epi, nd, nv, a, r, f, s);
-    } else {
-        launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
+    try {
+        A.online_begin(slot, trial);
+        if constexpr (std::is_same<T, double>::value) {
+            launch_spmv<double, double>(epi, nd, nv, a, r, f, s);
+        } else if (vp == Prec::BF16) {
+            launch_spmv<__hip_bfloat16, float>(epi, nd, nv, a, r, f, s);
+        } else {
+            launch_spmv<float, float>(epi, nd, nv, a, r, f, s);
+        }
+    } catch (...) {
+        kt.cancel(tok);
+        throw;
     }
     A.online_end(slot, trial);
     kt.end(tok);
@@ -474,7 +493,12 @@ void ew(int op, int nd, int nv, const EwArgs<T> &a, const RedArgs &r, const FinA
                  nv, (long long)a.n);
         tok = kt.begin(cls, ew_launch_bytes<T>(op, nd, nv, a));
     }
-    launch_ew<T>(op, nd, nv, a, r, f, Runtime::get().stream);
+    try {
+        launch_ew<T>(op, nd, nv, a, r, f, Runtime::get().stream);
+    } catch (...) {
+        kt.cancel(tok);
+        throw;
+    }
     kt.end(tok);
 }
 

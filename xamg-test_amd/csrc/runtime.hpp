@@ -205,9 +205,12 @@ class KernelTimer {
     };
     void start();
     void stop() { on_ = false; }
-    // token for end(), -1 when not recording
+    // token for end(), -1 when not recording; between begin() and end() the kernels
+    // launched on this thread stamp the class's start and stop events
     int begin(const std::string &cls, double bytes);
     void end(int token);
+    // drops the launch of `token` (its kernels were not launched)
+    void cancel(int token);
     // waits for the stream and folds the recorded launches into the table
     const std::vector<Stat> &stats();
 

@@ -607,10 +607,10 @@ void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const Fin
 #define XAMG_SPMV_CASE(E, D)                                                                       \
     if (epi == E && nd == D) {                                                                     \
         if constexpr ((STAGE == 3 || STAGE == 5) && sizeof(V) % 4 == 0)                            \
-            hipLaunchKernelGGL((k_spmv_pipe<V, T, NV, VPT, E, D, STAGE == 5>), g, b, 0, s, a, r, f,    \
+            XAMG_LAUNCH((k_spmv_pipe<V, T, NV, VPT, E, D, STAGE == 5>), g, b, 0, s, a, r, f,           \
                                c.xcd_remap);                                                        \
         else                                                                                       \
-            hipLaunchKernelGGL((k_spmv<V, T, NV, VPT, E, D, (STAGE == 3 ? 1 : STAGE == 5 ? 4 : STAGE)>), \
+            XAMG_LAUNCH((k_spmv<V, T, NV, VPT, E, D, (STAGE == 3 ? 1 : STAGE == 5 ? 4 : STAGE)>),    \
                                g, b, 0, s, a, r, f, c.xcd_remap);                                   \
         launch_finish<T, NV>(D, r, f, G, s);                                                       \
         return;                                                                                    \

@@ -180,7 +180,7 @@ static void spmv_nzp_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r,
     dim3 g(G), b(kBlock);
 #define XAMG_NZP_CASE(E, D)                                                                        \
     if (epi == E && nd == D) {                                                                     \
-        hipLaunchKernelGGL((k_spmv_nzp<V, T, NV, GV, E, D>), g, b, 0, s, a, r, f, c.xcd_remap);    \
+        XAMG_LAUNCH((k_spmv_nzp<V, T, NV, GV, E, D>), g, b, 0, s, a, r, f, c.xcd_remap);           \
         launch_finish<T, NV>(D, r, f, G, s);                                                       \
         return;                                                                                    \
     }

@@ -105,6 +105,7 @@ def lib():
             "xamgx_dot_noalloc": (ci, [sz, sz, ci, vp]),
             "xamgx_bench_spmv": (ci, [sz, ci, ci, ci, vp]),
             "xamgx_bench_copy": (ci, [C.c_uint64, ci, vp]),
+            "xamgx_bench_triad": (ci, [C.c_uint64, ci, vp]),
             "xamgx_bench_spmv_epi": (ci, [sz, ci, ci, ci, ci, ci, ci, vp, vp]),
             "xamgx_kernel_timing": (ci, [ci]),
             "xamgx_kernel_stats_count": (ci, [vp]),
@@ -594,6 +595,13 @@ def set_spmv_stage(mode):
 def bench_copy(nbytes, iters=20):
     g = C.c_double()
     check(lib().xamgx_bench_copy(nbytes, iters, C.byref(g)))
+    return g.value
+
+
+def bench_triad(nbytes, iters=20):
+    """STREAM triad a = b + s c over FP64 arrays of nbytes each: best GB/s of 3 x nbytes."""
+    g = C.c_double()
+    check(lib().xamgx_bench_triad(nbytes, iters, C.byref(g)))
     return g.value
 
 
