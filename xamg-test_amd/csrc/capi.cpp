@@ -1265,6 +1265,9 @@ int xamgx_batch_setup(XAMG_batch_id bid, XAMG_param_id pid) {
     return guard([&] {
         Batch *B = slot(g_batches, bid, "batch");
         GlobalParams *p = slot(g_params, pid, "param");
+        // the previous solver (its graph, pinned slots and vectors) goes first, so two
+        // solvers of one batch never coexist; a setup that fails leaves none
+        B->solver.reset();
         B->solver = make_batch_solver(*B, *p);
     });
 }
