@@ -197,7 +197,9 @@ int xamgx_batch_create(int nsys, const int *sys_rows, const int *nv, const int *
  * "preconditioner" (max_iters sweeps, relax_factor) in FP64 (pbicgstab.inl semantics),
  * or under a "meta_solver" IterativeRefinement with subsolver_reduced_precision: FP32
  * values and recurrences, FP64 solution and residual replacement, convergence on the
- * FP64 residual (replace_step as for the single-system solver). */
+ * FP64 residual (replace_step, default 1e-3 here; recurrence_operator fp64 (default) or
+ * fp32 for the recurrence products; preconditioner_operator fp32 (default) or fp64 for
+ * the Jacobi sweeps). */
 int xamgx_batch_setup(XAMG_batch_id bid, XAMG_param_id pid);
 int xamgx_batch_solve(XAMG_batch_id bid, XAMG_vector_id x, XAMG_vector_id b);
 /* Per system: Krylov iterations and FP64 residual checks (mixed precision); per RHS in
@@ -207,9 +209,20 @@ int xamgx_batch_get_info(XAMG_batch_id bid, int *iters, int *checks, int *conver
 int xamgx_batch_residuals(XAMG_batch_id bid, XAMG_vector_id x, XAMG_vector_id b, double *relres);
 /* Device time (ms) of the last batch solve. */
 int xamgx_batch_last_ms(XAMG_batch_id bid, double *ms);
-/* Workgroups of the persistent grid, rows per tile, iterations per host poll, hipGraph
- * replay (0/1); values <= 0 (< 0 for use_graph) keep the current setting. */
+/* Workgroups of the grid, rows per tile, iterations per host poll, hipGraph replay (0/1);
+ * values <= 0 (< 0 for use_graph) keep the current setting.  The tile is the block over
+ * which dot products are summed first (default 1024 rows), so it fixes their order.  A
+ * grid > 0 also switches the autotuner off (one tile per workgroup step, one element per
+ * SpMV lane). */
 int xamgx_batch_set_launch(int grid, int tile_rows, int batch, int use_graph);
+/* Autotuner of the launch shape (on by default): at setup, and again whenever the tile
+ * size changes, it times workgroups {2048, ..., 32768} x tiles per workgroup step
+ * {1, 2, 4} x elements per SpMV lane {1, 2} on the batch itself and keeps the fastest.
+ * None of these changes a result.  With on = 0, tps and ept (> 0) fix the last two. */
+int xamgx_batch_set_tuning(int on, int tps, int ept);
+/* Launch shape of the batch's solver: workgroups, tiles per step, elements per SpMV lane,
+ * and 1 if the autotuner chose it. */
+int xamgx_batch_get_launch(XAMG_batch_id bid, int *grid, int *tps, int *ept, int *tuned);
 int xamgx_batch_destroy(XAMG_batch_id bid);
 /* Workload generator: nsys 2D 5-point variable-coefficient systems, side
  * round(smin (smax/smin)^U_k) and nv_k = 1 + floor(maxnv V_k) with U_k, V_k the

@@ -59,9 +59,11 @@ def _run(data, params, b, grid=None, tile_rows=None, use_graph=None):
     it, ck, conv = B.info()
     out = {"x": x.download(), "iters": it, "checks": ck, "converged": conv,
            "relres": B.residuals(x, bv), "ms": B.last_ms()}
+    out["launch"] = B.launch()
     for o in (x, bv, B):
         o.destroy()
-    X.batch_set_launch(2048, 512, 8, 1)
+    X.batch_set_launch(0, 1024, 8, 1)
+    X.batch_set_tuning(1)
     return out
 
 
@@ -114,23 +116,37 @@ def test_batch_matches_single_system_solver(xamg, small):
         assert np.linalg.norm(gpu["x"][off[k]:off[k + 1]] - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
-def test_batch_launch_shape_and_graph_invariance(xamg, small):
-    """Results do not depend on the persistent grid size or on graph replay (the tile
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_launch_shape_and_graph_invariance(xamg, small, mixed):
+    """Results do not depend on the grid size, the tiles a workgroup takes per step, the
+    elements a SpMV lane keeps in flight, the autotuner's choice or graph replay (the tile
     partials and their order are fixed by the tiling)."""
     data, b = small
-    params = _params("PBiCGStab")
+    params = _params("PBiCGStab", mixed=mixed)
     a = _run(data, params, b, grid=2048, use_graph=1)
+    assert a["launch"] == (2048, 1, 1, 0)
     c = _run(data, params, b, grid=3, use_graph=0)
-    assert np.array_equal(a["x"], c["x"])
-    assert np.array_equal(a["iters"], c["iters"])
+    t = _run(data, params, b)
+    assert t["launch"][3] == 1 and t["launch"][0] in (2048, 4096, 8192, 16384, 32768)
+    X.batch_set_tuning(0, 4, 2)
+    f = _run(data, params, b, grid=5, use_graph=1)
+    for o in (c, t, f):
+        assert np.array_equal(a["x"], o["x"])
+        assert np.array_equal(a["iters"], o["iters"])
+        assert np.array_equal(a["checks"], o["checks"])
     d = _run(data, params, b, tile_rows=64)
     assert np.abs(d["iters"] - a["iters"]).max() <= 2
 
 
-def test_batch_mixed_refinement(xamg, small):
+@pytest.mark.parametrize("op", ["fp64", "fp32"])
+def test_batch_mixed_refinement(xamg, small, op):
+    """FP32 recurrences with FP64 residual replacement, their products with the FP64 or
+    the FP32 operator: FP64-verified residuals and the FP64 solution."""
     data, b = small
     ref = _run(data, _params("PBiCGStab", tol=1e-13), b)
-    mx = _run(data, _params("PBiCGStab", mixed=True), b)
+    p = _params("PBiCGStab", mixed=True)
+    p["meta_solver"]["recurrence_operator"] = op
+    mx = _run(data, p, b)
     assert mx["converged"].all()
     assert mx["relres"].max() <= 1.5e-10
     assert np.linalg.norm(mx["x"] - ref["x"]) <= 1e-7 * np.linalg.norm(ref["x"])

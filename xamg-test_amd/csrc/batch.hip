@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 
 namespace xamg {
 
@@ -15,6 +16,10 @@ BatchCfg &batch_cfg() {
 namespace {
 
 constexpr int BT = 256;      // threads per workgroup
+// Second Jacobi sweep from a zero start, with the first folded in: the gathered and the
+// row's own first-sweep values z1_j = (w d_j) q_j are formed from q as the first sweep
+// computes them (EW_JACOBI0), so no z1 vector is written or read.
+constexpr int BEPI_JACOBI2 = 100;
 constexpr int PSTRIDE = kBatchDots * kBatchMaxNV; // partial sums per tile
 
 template <typename T>
@@ -25,6 +30,7 @@ struct BSpmvArgs {
     const T *b = nullptr;
     const T *xin = nullptr;
     const T *d = nullptr;     // inverse diagonal per global row
+    const double *d64 = nullptr; // FP64 inverse diagonal (Jacobi sweeps with FP64 values)
     const T *w = nullptr;     // dot partner
     const T *wcoef = nullptr; // JACOBI relax factors, per (system, RHS)
     float *yo = nullptr, *yo2 = nullptr, *yo3 = nullptr; // EPI_RESID_MX outputs
@@ -61,15 +67,18 @@ struct BFinArgs {
 };
 
 // Sum of the per-thread accumulators of one tile over the workgroup, in a fixed order,
-// stored as the tile's partial sums by thread 0.
+// stored as the tile's partial sums by thread 0 (slots of RHS the system does not have
+// are stored as 0).
 template <int ND>
-__device__ void tile_partial(double (&acc)[ND][kBatchMaxNV], double *out) {
+__device__ void tile_partial(double (&acc)[ND][kBatchMaxNV], int nv, double *out) {
     __shared__ double lds[BT / 64][ND][kBatchMaxNV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < ND; k++)
 #pragma unroll
         for (int l = 0; l < kBatchMaxNV; l++) {
+            if (l >= nv) // nv is the same for the whole workgroup
+                break;
             double v = acc[k][l];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1)
@@ -81,9 +90,12 @@ __device__ void tile_partial(double (&acc)[ND][kBatchMaxNV], double *out) {
     if (threadIdx.x == 0)
         for (int k = 0; k < ND; k++)
             for (int l = 0; l < kBatchMaxNV; l++) {
-                double v = lds[0][k][l];
-                for (int q = 1; q < BT / 64; q++)
-                    v += lds[q][k][l];
+                double v = 0.0;
+                if (l < nv) {
+                    v = lds[0][k][l];
+                    for (int q = 1; q < BT / 64; q++)
+                        v += lds[q][k][l];
+                }
                 out[k * kBatchMaxNV + l] = v;
             }
     __syncthreads();
@@ -122,101 +134,180 @@ __device__ __forceinline__ void acc_slot(double (&acc)[kBatchMaxNV], int l, doub
 // (row, RHS): lanes of one row read the same column / value entries; the row's nonzeros
 // are loaded U at a time and summed in ascending column order (csr_matvec.inl:213-231,
 // x * val).  U is the smallest compiled chunk that holds the batch's longest row (5 for
-// 5-point stencils), so a row costs one round of loads and no padding loads.
-template <typename V, typename T, int EPI, int ND, int U>
+// 5-point stencils), so a row costs one round of loads and no padding loads.  A lane
+// keeps EPT elements (BT apart) in flight: their row operands, nonzeros and gathers are
+// all loaded before the first sum, and the lane adds them to its dot accumulators in
+// element order, so the dots do not depend on EPT.  A workgroup takes L.tps consecutive
+// tiles per step of the grid stride (tiles are the dot blocks: neither changes a sum).
+// With FP64 values and FP32 vectors (the recurrence products of the FP32 solver with the
+// FP64 operator) the row sums run in FP64 and are rounded once, at the store.
+template <typename V, typename T, int EPI, int ND, int U, int EPT>
 __global__ void __launch_bounds__(BT)
     kb_spmv(BLayout L, const int *halt, BSpmvArgs<T> a, double *part) {
+    using Acc = typename std::conditional<(sizeof(V) > sizeof(T)), V, T>::type;
+    constexpr bool kWide = !std::is_same<Acc, T>::value;
+    constexpr bool kJac = EPI == EPI_JACOBI || EPI == BEPI_JACOBI2;
     const V *val = (const V *)a.val;
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
-        const int s = L.tsys[t];
-        if ((halt != nullptr && halt[s]) || (a.gate != nullptr && !a.gate[s]))
-            continue;
-        const int nv = L.nv[s];
-        const uint32_t rs = L.srow[s];
-        const int64_t vb = L.voff[s] - (int64_t)rs * nv;
-        const uint32_t q0 = (L.trow[t] - rs) * nv, q1 = (L.trow[t + 1] - rs) * nv;
-        const bool cp = a.copy_if == nullptr || a.copy_if[s];
-        constexpr int NDX = ND > 0 ? ND : 1;
-        double dacc[NDX][kBatchMaxNV];
+    const int tps = L.tps;
+    for (int t0 = blockIdx.x * tps; t0 < L.ntiles; t0 += gridDim.x * tps) {
+        const int t1 = min(L.ntiles, t0 + tps);
+        for (int t = t0; t < t1; t++) {
+            const int s = L.tsys[t];
+            if ((halt != nullptr && halt[s]) || (a.gate != nullptr && !a.gate[s]))
+                continue;
+            const int nv = L.nv[s];
+            const uint32_t rs = L.srow[s];
+            const int64_t eb = L.voff[s];
+            const int64_t vb = eb - (int64_t)rs * nv;
+            const uint32_t q0 = (L.trow[t] - rs) * nv, q1 = (L.trow[t + 1] - rs) * nv;
+            const bool cp = a.copy_if == nullptr || a.copy_if[s];
+            constexpr int NDX = ND > 0 ? ND : 1;
+            double dacc[NDX][kBatchMaxNV];
 #pragma unroll
-        for (int k = 0; k < NDX; k++)
+            for (int k = 0; k < NDX; k++)
 #pragma unroll
-            for (int l = 0; l < kBatchMaxNV; l++)
-                dacc[k][l] = 0.0;
-        for (uint32_t q = q0 + threadIdx.x; q < q1; q += BT) {
-            const uint32_t li = div_nv(q, nv);
-            const int l = (int)(q - li * nv);
-            const uint32_t i = rs + li;
-            const int64_t e = vb + (int64_t)rs * nv + q;
-            const T *xb = a.x + vb + l;
-            T acc = (T)0;
-            // the epilogue's and the dots' row operands are loaded before the gathers, so
-            // their latency overlaps the nonzero loop
-            T pb = (T)0, px = (T)0, pd = (T)0, pw = (T)0;
-            if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX || EPI == EPI_JACOBI)
-                pb = a.b[e];
-            if constexpr (EPI == EPI_JACOBI) {
-                px = a.xin[e];
-                pd = a.d[i];
-            }
-            if constexpr (ND == 3)
-                pw = a.w[e];
-            else if constexpr (ND == 1)
-                pw = a.w != nullptr ? a.w[e] : (T)0;
-            const uint32_t j0 = L.row[i], j1 = L.row[i + 1];
-            for (uint32_t j = j0; j < j1; j += U) {
-                uint32_t c[U];
-                T v[U], xv[U];
+                for (int l = 0; l < kBatchMaxNV; l++)
+                    dacc[k][l] = 0.0;
+            for (uint32_t qb = q0 + threadIdx.x; qb < q1; qb += EPT * BT) {
+                bool on[EPT];
+                int l[EPT];
+                uint32_t i[EPT], j0[EPT], n[EPT];
+                int64_t e[EPT];
+                // the epilogue's and the dots' row operands are loaded before the gathers,
+                // so their latency overlaps the nonzero loads
+                T pb[EPT], px[EPT], pw[EPT];
+                Acc pd[EPT], wl[EPT]; // Jacobi: inverse diagonal and relax factor
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const bool ok = j + u < j1;
-                    c[u] = ok ? L.col[j + u] : L.col[j0];
-                    v[u] = ok ? (T)val[j + u] : (T)0;
+                for (int k = 0; k < EPT; k++) {
+                    on[k] = qb + k * BT < q1;
+                    // an element past the tile stands in for the step's first element
+                    const uint32_t q = on[k] ? qb + k * BT : qb;
+                    const uint32_t li = div_nv(q, nv);
+                    l[k] = (int)(q - li * nv);
+                    i[k] = rs + li;
+                    e[k] = eb + q;
+                    pb[k] = px[k] = pw[k] = (T)0;
+                    pd[k] = wl[k] = (Acc)0;
+                    if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX || kJac)
+                        pb[k] = a.b[e[k]];
+                    if constexpr (EPI == EPI_JACOBI)
+                        px[k] = a.xin[e[k]];
+                    if constexpr (kJac) {
+                        if constexpr (kWide)
+                            pd[k] = a.d64[i[k]];
+                        else
+                            pd[k] = a.d[i[k]];
+                        wl[k] = (Acc)a.wcoef[s * kBatchMaxNV + l[k]];
+                    }
+                    if constexpr (ND == 3)
+                        pw[k] = a.w[e[k]];
+                    else if constexpr (ND == 1)
+                        pw[k] = a.w != nullptr ? a.w[e[k]] : (T)0;
+                    j0[k] = L.row[i[k]];
+                    n[k] = L.row[i[k] + 1] - j0[k];
+                }
+                uint32_t nmax = n[0];
+#pragma unroll
+                for (int k = 1; k < EPT; k++)
+                    nmax = max(nmax, n[k]);
+                Acc acc[EPT];
+#pragma unroll
+                for (int k = 0; k < EPT; k++)
+                    acc[k] = (Acc)0;
+                for (uint32_t jo = 0; jo < nmax; jo += U) {
+                    uint32_t c[EPT][U];
+                    Acc v[EPT][U], xa[EPT][U];
+                    T xv[EPT][U];
+#pragma unroll
+                    for (int k = 0; k < EPT; k++)
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const bool ok = jo + u < n[k];
+                            // a padding slot gathers the row's own element (in its system)
+                            c[k][u] = ok ? L.col[j0[k] + jo + u] : i[k];
+                            v[k][u] = ok ? (Acc)val[j0[k] + jo + u] : (Acc)0;
+                        }
+#pragma unroll
+                    for (int k = 0; k < EPT; k++)
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                            xv[k][u] = a.x[vb + (int64_t)c[k][u] * nv + l[k]];
+                    if constexpr (EPI == BEPI_JACOBI2) {
+                        Acc dv[EPT][U];
+#pragma unroll
+                        for (int k = 0; k < EPT; k++)
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                if constexpr (kWide)
+                                    dv[k][u] = a.d64[c[k][u]];
+                                else
+                                    dv[k][u] = a.d[c[k][u]];
+                            }
+#pragma unroll
+                        for (int k = 0; k < EPT; k++)
+#pragma unroll
+                            for (int u = 0; u < U; u++)
+                                xa[k][u] = (wl[k] * dv[k][u]) * (Acc)xv[k][u];
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < EPT; k++)
+#pragma unroll
+                            for (int u = 0; u < U; u++)
+                                xa[k][u] = (Acc)xv[k][u];
+                    }
+#pragma unroll
+                    for (int k = 0; k < EPT; k++)
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                            if (jo + u < n[k])
+                                acc[k] = acc[k] + xa[k][u] * v[k][u];
                 }
 #pragma unroll
-                for (int u = 0; u < U; u++)
-                    xv[u] = xb[(int64_t)c[u] * nv];
-#pragma unroll
-                for (int u = 0; u < U; u++)
-                    if (j + u < j1)
-                        acc = acc + xv[u] * v[u];
-            }
-            T out;
-            if constexpr (EPI == EPI_AXY) {
-                out = acc;
-            } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
-                out = (T)1.0 * pb + (T)(-1.0) * acc;
-            } else { // EPI_JACOBI: x_new = x + (w d)(b - A x)
-                out = px + (a.wcoef[s * kBatchMaxNV + l] * pd) * (pb - acc);
-            }
-            if constexpr (EPI == EPI_RESID_MX) {
-                const float o = (float)out;
-                a.yo[e] = o;
-                if (cp) {
-                    if (a.yo2 != nullptr)
-                        a.yo2[e] = o;
-                    if (a.yo3 != nullptr)
-                        a.yo3[e] = o;
+                for (int k = 0; k < EPT; k++) {
+                    if (!on[k])
+                        break;
+                    const T ax = (T)acc[k];
+                    T out;
+                    if constexpr (EPI == EPI_AXY) {
+                        out = ax;
+                    } else if constexpr (EPI == EPI_RESID || EPI == EPI_RESID_MX) {
+                        out = (T)1.0 * pb[k] + (T)(-1.0) * ax;
+                    } else if constexpr (EPI == EPI_JACOBI) { // x_new = x + (w d)(b - A x)
+                        out = (T)((Acc)px[k] + (wl[k] * pd[k]) * ((Acc)pb[k] - acc[k]));
+                    } else { // BEPI_JACOBI2: x1 = (w d) b, then x_new = x1 + (w d)(b - A x1)
+                        const Acc x1 = (wl[k] * pd[k]) * (Acc)pb[k];
+                        out = (T)(x1 + (wl[k] * pd[k]) * ((Acc)pb[k] - acc[k]));
+                    }
+                    if constexpr (EPI == EPI_RESID_MX) {
+                        const float o = (float)out;
+                        a.yo[e[k]] = o;
+                        if (cp) {
+                            if (a.yo2 != nullptr)
+                                a.yo2[e[k]] = o;
+                            if (a.yo3 != nullptr)
+                                a.yo3[e[k]] = o;
+                        }
+                    } else {
+                        a.y[e[k]] = out;
+                        if (a.y2 != nullptr)
+                            a.y2[e[k]] = out;
+                        if (a.y3 != nullptr)
+                            a.y3[e[k]] = out;
+                    }
+                    if constexpr (ND == 1) {
+                        const double wv = a.w != nullptr ? (double)pw[k] : (double)out;
+                        acc_slot(dacc[0], l[k], (double)out * wv);
+                    } else if constexpr (ND == 3) {
+                        const double o = (double)out, wv = (double)pw[k];
+                        acc_slot(dacc[0], l[k], o * wv);
+                        acc_slot(dacc[1], l[k], o * o);
+                        acc_slot(dacc[2], l[k], wv * wv);
+                    }
                 }
-            } else {
-                a.y[e] = out;
-                if (a.y2 != nullptr)
-                    a.y2[e] = out;
-                if (a.y3 != nullptr)
-                    a.y3[e] = out;
             }
-            if constexpr (ND == 1) {
-                const double wv = a.w != nullptr ? (double)pw : (double)out;
-                acc_slot(dacc[0], l, (double)out * wv);
-            } else if constexpr (ND == 3) {
-                const double o = (double)out, wv = (double)pw;
-                acc_slot(dacc[0], l, o * wv);
-                acc_slot(dacc[1], l, o * o);
-                acc_slot(dacc[2], l, wv * wv);
-            }
+            if constexpr (ND > 0)
+                tile_partial<ND>(dacc, nv, part + (int64_t)t * PSTRIDE);
         }
-        if constexpr (ND > 0)
-            tile_partial<ND>(dacc, part + (int64_t)t * PSTRIDE);
     }
 }
 
@@ -225,7 +316,9 @@ __global__ void __launch_bounds__(BT)
 template <typename T, int OP, int ND>
 __global__ void __launch_bounds__(BT)
     kb_ew(BLayout L, const int *halt, BEwArgs<T> a, double *part) {
-    for (int t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
+    const int tps = L.tps;
+    for (int t0 = blockIdx.x * tps; t0 < L.ntiles; t0 += gridDim.x * tps)
+    for (int t = t0; t < min(L.ntiles, t0 + tps); t++) {
         const int s = L.tsys[t];
         if (halt != nullptr && halt[s])
             continue;
@@ -353,7 +446,7 @@ __global__ void __launch_bounds__(BT)
             }
         }
         if constexpr (ND > 0)
-            tile_partial<ND>(dacc, part + (int64_t)t * PSTRIDE);
+            tile_partial<ND>(dacc, nv, part + (int64_t)t * PSTRIDE);
     }
 }
 
@@ -780,24 +873,34 @@ template const float *Batch::inv_diag<float>();
 // ---------------------------------------------------------------------------
 namespace {
 
+template <typename V, typename T, int EPI, int ND, int U>
+void launch_bspmv_u(const BLayout &L, int G, const int *halt, const BSpmvArgs<T> &a, double *part,
+                    hipStream_t st) {
+    if (L.ept == 2)
+        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, U, 2>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+    else
+        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, U, 1>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+}
+int grid_of(const BLayout &L) {
+    return std::max(1, std::min(L.grid, (L.ntiles + L.tps - 1) / L.tps));
+}
 template <typename V, typename T, int EPI, int ND>
 void launch_bspmv(const BLayout &L, const int *halt, const BSpmvArgs<T> &a, double *part,
                   hipStream_t st) {
-    const int G = std::max(1, std::min(batch_cfg().grid, L.ntiles));
+    const int G = grid_of(L);
     if (L.max_row_nnz <= 3)
-        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 3>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+        launch_bspmv_u<V, T, EPI, ND, 3>(L, G, halt, a, part, st);
     else if (L.max_row_nnz <= 5)
-        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 5>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+        launch_bspmv_u<V, T, EPI, ND, 5>(L, G, halt, a, part, st);
     else if (L.max_row_nnz <= 7)
-        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 7>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+        launch_bspmv_u<V, T, EPI, ND, 7>(L, G, halt, a, part, st);
     else
-        hipLaunchKernelGGL((kb_spmv<V, T, EPI, ND, 9>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+        launch_bspmv_u<V, T, EPI, ND, 9>(L, G, halt, a, part, st);
 }
 template <typename T, int OP, int ND>
 void launch_bew(const BLayout &L, const int *halt, const BEwArgs<T> &a, double *part,
                 hipStream_t st) {
-    const int G = std::max(1, std::min(batch_cfg().grid, L.ntiles));
-    hipLaunchKernelGGL((kb_ew<T, OP, ND>), dim3(G), dim3(BT), 0, st, L, halt, a, part);
+    hipLaunchKernelGGL((kb_ew<T, OP, ND>), dim3(grid_of(L)), dim3(BT), 0, st, L, halt, a, part);
 }
 template <typename F>
 void launch_bfin(const BLayout &L, const BState<F> &S, const BFinArgs &f, hipStream_t st) {
@@ -815,14 +918,28 @@ class BatchKrylov : public BatchKrylovBase {
     double relax = 0.99;
     int max_iters = 50;
     double abs_tol = 0, rel_tol = 0, delta2 = 1e-4, theta = 1e-7, gap2 = 100.0;
+    // FP32 solver: the products v = A p^ and t = A s^ of the recurrences read the FP64
+    // values and sum in FP64 (key recurrence_operator, default fp64): the recurrence then
+    // tracks b - A x for the operator itself, not for its FP32 rounding, and the
+    // ill-conditioned systems need 10% fewer row-iterations; the Jacobi sweeps keep the
+    // FP32 values
+    bool op64 = true;
+    // FP32 solver: the Jacobi sweeps of two or more read the FP64 values and inverse
+    // diagonal and compute in FP64, rounding each sweep's result once (key
+    // preconditioner_operator = fp64)
+    bool pre64 = false;
     std::vector<DBuf> vec;
     DBuf sc, cv, status, sy, ndone, part;
     int *pinned = nullptr;
     BState<T> S;
     hipGraphExec_t gexec = nullptr;
     const double *g_x = nullptr, *g_b = nullptr;
-    int g_k = 0, g_tiles = 0, g_grid = 0;
+    int g_k = 0, g_tiles = 0, g_grid = 0, g_tps = 0, g_ept = 0;
     int g_gen = -1; // tile-array generation the graph's launches point into
+    // launch shape: workgroups, tiles per workgroup step, elements per SpMV lane; tuned
+    // for the tile-array generation t_gen
+    int lc_grid = 8192, lc_tps = 1, lc_ept = 1, t_gen = -1;
+    bool tuned = false;
 
     explicit BatchKrylov(Batch &b) : B(b) {}
     ~BatchKrylov() override {
@@ -855,14 +972,24 @@ class BatchKrylov : public BatchKrylovBase {
         rel_tol = top.get_float("rel_tolerance");
         max_iters = (int)sp.get_int("max_iters");
         if (kMixed) {
-            // replacement after each 10x drop of the recurrence residual: over three
-            // 2,000-system batches 3-10% faster than 100x (fewer FP32 iterations on the
-            // ill-conditioned systems for a few more FP64 checks)
-            const double step = top.has("replace_step") ? top.get_float("replace_step") : 1e-1;
+            // replacement after each 1000x drop of the recurrence residual: with the FP64
+            // recurrence operator, over three 2,000-system batches 2-3% faster than 10x or
+            // 100x and 2-5% faster than 1e4x or 1e6x
+            const double step = top.has("replace_step") ? top.get_float("replace_step") : 1e-3;
             delta2 = step * step;
             theta = top.has("restart_threshold") ? top.get_float("restart_threshold") : 1e-7;
             const double gap = top.has("restart_gap") ? top.get_float("restart_gap") : 10.0;
             gap2 = gap * gap;
+            if (top.has("recurrence_operator")) {
+                const std::string o = top.get_string("recurrence_operator");
+                XAMG_REQUIRE(o == "fp64" || o == "fp32", "batch: recurrence_operator must be fp64 or fp32");
+                op64 = o == "fp64";
+            }
+            if (top.has("preconditioner_operator")) {
+                const std::string o = top.get_string("preconditioner_operator");
+                XAMG_REQUIRE(o == "fp64" || o == "fp32", "batch: preconditioner_operator must be fp64 or fp32");
+                pre64 = o == "fp64";
+            }
         }
         const int nb = pre ? 9 : 6;
         vec.clear();
@@ -877,11 +1004,136 @@ class BatchKrylov : public BatchKrylovBase {
         XAMG_HIP_CHECK(hipHostMalloc((void **)&pinned, 2 * sizeof(int), hipHostMallocDefault));
         B.values(kMixed);
         B.inv_diag<T>();
+        if (kMixed && pre64)
+            B.inv_diag<double>();
         S.sc = sc.as<T>();
         S.cv = cv.as<double>();
         S.status = status.as<int>();
         S.sy = sy.as<int>();
         S.ndone = ndone.as<int>();
+        prepare(Runtime::get().stream);
+    }
+
+    BLayout lay() const {
+        BLayout L = B.layout();
+        L.grid = lc_grid;
+        L.tps = lc_tps;
+        L.ept = lc_ept;
+        return L;
+    }
+
+    // Tile arrays for the configured tile size, the partial-sum buffer, and the launch
+    // shape: the configured one, or the autotuner's for this tile-array generation.
+    void prepare(hipStream_t st) {
+        const BatchCfg &cfg = batch_cfg();
+        B.set_tiles(cfg.tile_rows);
+        if (part.bytes < sizeof(double) * PSTRIDE * (size_t)B.ntiles)
+            part.alloc(sizeof(double) * PSTRIDE * (size_t)B.ntiles);
+        S.part = part.as<double>();
+        if (!cfg.tune) {
+            lc_grid = std::max(1, cfg.grid);
+            lc_tps = std::max(1, cfg.tps);
+            lc_ept = cfg.ept == 2 ? 2 : 1;
+            tuned = false;
+        } else if (!tuned || t_gen != B.gen) {
+            tune(st);
+        }
+    }
+
+    // Times the launch shapes on the kernels that dominate an iteration while every system
+    // runs (the three-dot product t = A s^, the folded Jacobi sweeps, the r update with its
+    // dots) and keeps the fastest.  None of the shapes changes a result: tiles fix the dot
+    // order, and a lane's elements are summed in element order whatever it keeps in flight.
+    void tune(hipStream_t st) {
+        const int NS4 = B.nsys * kBatchMaxNV;
+        hipLaunchKernelGGL((kb_state_init<T>), dim3((NS4 + 255) / 256), dim3(256), 0, st, S,
+                           B.nsys, (T)relax);
+        BLayout L = lay();
+        for (size_t k = 0; k < vec.size(); k++) {
+            BEwArgs<T> a;
+            a.z = V((int)k);
+            a.x = V((int)k);
+            a.c0 = (T)1;
+            launch_bew<T, EW_FILL, 0>(L, nullptr, a, nullptr, st);
+        }
+        double *pt = part.as<double>();
+        auto round = [&](const BLayout &Lc) {
+            BSpmvArgs<T> m;
+            m.x = V(4);
+            m.y = V(5);
+            m.w = V(4);
+            if (op64) {
+                m.val = B.values(false);
+                launch_bspmv<double, T, EPI_AXY, 3>(Lc, nullptr, m, pt, st);
+            } else {
+                m.val = B.values(kMixed);
+                launch_bspmv<T, T, EPI_AXY, 3>(Lc, nullptr, m, pt, st);
+            }
+            if (pre) {
+                BSpmvArgs<T> j;
+                j.x = V(2);
+                j.b = V(2);
+                j.y = V(6);
+                j.d = B.inv_diag<T>();
+                j.wcoef = scf(BS_RELAX);
+                if (kMixed && pre64) {
+                    j.val = B.values(false);
+                    j.d64 = B.inv_diag<double>();
+                    launch_bspmv<double, T, BEPI_JACOBI2, 0>(Lc, nullptr, j, nullptr, st);
+                } else {
+                    j.val = B.values(kMixed);
+                    launch_bspmv<T, T, BEPI_JACOBI2, 0>(Lc, nullptr, j, nullptr, st);
+                }
+            }
+            BEwArgs<T> r;
+            r.z = V(1);
+            r.x = V(4);
+            r.y = V(5);
+            r.cb = scf(BS_NOMEGA);
+            r.w = V(0);
+            launch_bew<T, EW_AXPBY_Z, 3>(Lc, nullptr, r, pt, st);
+        };
+        hipEvent_t e0, e1;
+        XAMG_HIP_CHECK(hipEventCreate(&e0));
+        XAMG_HIP_CHECK(hipEventCreate(&e1));
+        float best = 0;
+        for (int g : {2048, 4096, 8192, 16384, 32768})
+            for (int tp : {1, 2, 4})
+                for (int ep : {1, 2}) {
+                    BLayout Lc = L;
+                    Lc.grid = g;
+                    Lc.tps = tp;
+                    Lc.ept = ep;
+                    round(Lc);
+                    XAMG_HIP_CHECK(hipEventRecord(e0, st));
+                    for (int r = 0; r < 3; r++)
+                        round(Lc);
+                    XAMG_HIP_CHECK(hipEventRecord(e1, st));
+                    XAMG_HIP_CHECK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    XAMG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    if (best == 0 || ms < best) {
+                        best = ms;
+                        lc_grid = g;
+                        lc_tps = tp;
+                        lc_ept = ep;
+                    }
+                }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tuned = true;
+        t_gen = B.gen;
+    }
+
+    void launch(int *grid, int *tps, int *ept, int *was_tuned) const override {
+        if (grid)
+            *grid = lc_grid;
+        if (tps)
+            *tps = lc_tps;
+        if (ept)
+            *ept = lc_ept;
+        if (was_tuned)
+            *was_tuned = tuned ? 1 : 0;
     }
 
     const int *halt() const { return S.sy + BY_HALT * B.nsys; }
@@ -905,17 +1157,34 @@ class BatchKrylov : public BatchKrylovBase {
     void jacobi(const T *q, T *out, T *tmp, hipStream_t st, const BLayout &L) {
         const T *d = B.inv_diag<T>();
         const T *src = nullptr;
-        {
-            T *dst = ((sweeps - 1) % 2 == 0) ? out : tmp;
+        if (sweeps == 1) {
             BEwArgs<T> a;
-            a.z = dst;
+            a.z = out;
             a.x = q;
             a.d = d;
             a.ca = scf(BS_RELAX);
             launch_bew<T, EW_JACOBI0, 0>(L, halt(), a, nullptr, st);
+            return;
+        }
+        { // sweeps 1 and 2 in one launch
+            T *dst = ((sweeps - 2) % 2 == 0) ? out : tmp;
+            BSpmvArgs<T> a;
+            a.x = q;
+            a.y = dst;
+            a.b = q;
+            a.d = d;
+            a.wcoef = scf(BS_RELAX);
+            if (kMixed && pre64) {
+                a.val = B.values(false);
+                a.d64 = B.inv_diag<double>();
+                launch_bspmv<double, T, BEPI_JACOBI2, 0>(L, halt(), a, nullptr, st);
+            } else {
+                a.val = B.values(kMixed);
+                launch_bspmv<T, T, BEPI_JACOBI2, 0>(L, halt(), a, nullptr, st);
+            }
             src = dst;
         }
-        for (int k = 2; k <= sweeps; k++) {
+        for (int k = 3; k <= sweeps; k++) {
             T *dst = ((sweeps - k) % 2 == 0) ? out : tmp;
             BSpmvArgs<T> a;
             a.val = B.values(kMixed);
@@ -925,7 +1194,13 @@ class BatchKrylov : public BatchKrylovBase {
             a.b = q;
             a.d = d;
             a.wcoef = scf(BS_RELAX);
-            launch_bspmv<T, T, EPI_JACOBI, 0>(L, halt(), a, nullptr, st);
+            if (kMixed && pre64) {
+                a.val = B.values(false);
+                a.d64 = B.inv_diag<double>();
+                launch_bspmv<double, T, EPI_JACOBI, 0>(L, halt(), a, nullptr, st);
+            } else {
+                launch_bspmv<T, T, EPI_JACOBI, 0>(L, halt(), a, nullptr, st);
+            }
             src = dst;
         }
     }
@@ -971,7 +1246,12 @@ class BatchKrylov : public BatchKrylovBase {
             a.x = p_;
             a.y = v;
             a.w = r0;
-            launch_bspmv<T, T, EPI_AXY, 1>(L, halt(), a, pt, st);
+            if (op64) {
+                a.val = B.values(false);
+                launch_bspmv<double, T, EPI_AXY, 1>(L, halt(), a, pt, st);
+            } else {
+                launch_bspmv<T, T, EPI_AXY, 1>(L, halt(), a, pt, st);
+            }
             launch_bfin<T>(L, S, fin(FIN_ALPHA, 1), st);
         }
         { // s = r - alpha v
@@ -990,7 +1270,12 @@ class BatchKrylov : public BatchKrylovBase {
             a.x = s_;
             a.y = t;
             a.w = s;
-            launch_bspmv<T, T, EPI_AXY, 3>(L, halt(), a, pt, st);
+            if (op64) {
+                a.val = B.values(false);
+                launch_bspmv<double, T, EPI_AXY, 3>(L, halt(), a, pt, st);
+            } else {
+                launch_bspmv<T, T, EPI_AXY, 3>(L, halt(), a, pt, st);
+            }
             launch_bfin<T>(L, S, fin(kMixed ? FIN_MX_OMEGA : FIN_OMEGA, 3), st);
         }
         if constexpr (kMixed) {
@@ -1073,11 +1358,8 @@ class BatchKrylov : public BatchKrylovBase {
         Runtime &rt = Runtime::get();
         hipStream_t st = rt.stream;
         const BatchCfg &cfg = batch_cfg();
-        B.set_tiles(cfg.tile_rows);
-        if (part.bytes < sizeof(double) * PSTRIDE * (size_t)B.ntiles)
-            part.alloc(sizeof(double) * PSTRIDE * (size_t)B.ntiles);
-        S.part = part.as<double>();
-        const BLayout L = B.layout();
+        prepare(st);
+        const BLayout L = lay();
         const int K = std::max(1, cfg.batch);
         hipEvent_t e0, e1;
         XAMG_HIP_CHECK(hipEventCreate(&e0));
@@ -1087,7 +1369,7 @@ class BatchKrylov : public BatchKrylovBase {
         // one graph of K iterations, rebuilt when the vectors or the launch shape change
         if (cfg.use_graph &&
             (!gexec || g_x != x || g_b != b || g_k != K || g_tiles != B.ntiles ||
-             g_grid != cfg.grid || g_gen != B.gen)) {
+             g_grid != L.grid || g_tps != L.tps || g_ept != L.ept || g_gen != B.gen)) {
             if (gexec) {
                 (void)hipGraphExecDestroy(gexec);
                 gexec = nullptr;
@@ -1103,7 +1385,9 @@ class BatchKrylov : public BatchKrylovBase {
             g_b = b;
             g_k = K;
             g_tiles = B.ntiles;
-            g_grid = cfg.grid;
+            g_grid = L.grid;
+            g_tps = L.tps;
+            g_ept = L.ept;
             g_gen = B.gen;
         }
         hipEvent_t ev[2];

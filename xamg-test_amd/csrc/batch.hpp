@@ -66,6 +66,9 @@ struct BLayout {
     const int *stile = nullptr;     // system -> first tile [nsys+1]
     const uint32_t *row = nullptr, *col = nullptr;
     int max_row_nnz = 0; // longest row of the batch (fixes the SpMV's nonzero chunk)
+    int tps = 1;         // tiles a workgroup takes per step of the grid stride
+    // launch shape (host side): workgroups, elements a SpMV lane keeps in flight
+    int grid = 8192, ept = 1;
 };
 
 // Device view of the per-system solver state.
@@ -117,6 +120,9 @@ class BatchKrylovBase {
     // per-system iterations / inner iterations, per-RHS convergence flags (nelem order
     // of RHS: system-major)
     virtual void info(int *iters, int *sub_iters, int *converged) = 0;
+    // launch shape in use: workgroups, tiles per workgroup step, elements per lane of the
+    // SpMV, and whether the autotuner chose it
+    virtual void launch(int *grid, int *tps, int *ept, int *tuned) const = 0;
 };
 
 // Builds the batch solver from a parameter dictionary: "solver" BiCGStab / PBiCGStab
@@ -127,13 +133,20 @@ std::unique_ptr<BatchKrylovBase> make_batch_solver(Batch &B, const GlobalParams 
 // FP64 relative residuals ||b - A x|| / ||b|| per RHS (system-major order).
 void batch_residuals(Batch &B, const double *x, const double *b, double *relres);
 
-// Launch configuration of the batch kernels: workgroups of the persistent grid, rows per
-// tile, iterations per host poll, hipGraph replay.
+// Launch configuration of the batch kernels: workgroups of the grid, rows per tile,
+// iterations per host poll, hipGraph replay.  A tile is the block over which the dot
+// products are summed first, so tile_rows fixes their order; the workgroups, the tiles per
+// workgroup step and the elements a SpMV lane keeps in flight do not change any result
+// and are timed by the autotuner at setup unless `tune` is 0 (then grid, tps and ept
+// below are used).
 struct BatchCfg {
-    int grid = 2048;
-    int tile_rows = 512;
+    int grid = 8192;
+    int tile_rows = 1024;
     int batch = 8;
     int use_graph = 1;
+    int tune = 1;
+    int tps = 1;
+    int ept = 1;
 };
 BatchCfg &batch_cfg();
 

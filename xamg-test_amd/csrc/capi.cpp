@@ -1304,14 +1304,37 @@ int xamgx_batch_last_ms(XAMG_batch_id bid, double *ms) {
 int xamgx_batch_set_launch(int grid, int tile_rows, int batch, int use_graph) {
     return guard([&] {
         BatchCfg &c = batch_cfg();
-        if (grid > 0)
+        if (grid > 0) {
             c.grid = grid;
+            c.tune = 0; // a fixed grid: one element per lane and one tile per step
+            c.tps = 1;
+            c.ept = 1;
+        }
         if (tile_rows > 0)
             c.tile_rows = tile_rows;
         if (batch > 0)
             c.batch = batch;
         if (use_graph >= 0)
             c.use_graph = use_graph;
+    });
+}
+int xamgx_batch_set_tuning(int on, int tps, int ept) {
+    return guard([&] {
+        BatchCfg &c = batch_cfg();
+        XAMG_REQUIRE(tps >= 0 && (ept == 0 || ept == 1 || ept == 2),
+                     "batch: tiles per step must be >= 1 and elements per lane 1 or 2");
+        c.tune = on ? 1 : 0;
+        if (tps > 0)
+            c.tps = tps;
+        if (ept > 0)
+            c.ept = ept;
+    });
+}
+int xamgx_batch_get_launch(XAMG_batch_id bid, int *grid, int *tps, int *ept, int *tuned) {
+    return guard([&] {
+        Batch *B = slot(g_batches, bid, "batch");
+        XAMG_REQUIRE(B->solver != nullptr, "batch: call xamgx_batch_setup first");
+        B->solver->launch(grid, tps, ept, tuned);
     });
 }
 int xamgx_batch_destroy(XAMG_batch_id bid) {

@@ -137,6 +137,8 @@ def lib():
             "xamgx_batch_residuals": (ci, [sz, sz, sz, vp]),
             "xamgx_batch_last_ms": (ci, [sz, vp]),
             "xamgx_batch_set_launch": (ci, [ci, ci, ci, ci]),
+            "xamgx_batch_set_tuning": (ci, [ci, ci, ci]),
+            "xamgx_batch_get_launch": (ci, [sz, vp, vp, vp, vp]),
             "xamgx_batch_destroy": (ci, [sz]),
             "xamgx_gen_batch_var5_layout": (ci, [ci, C.c_uint32, ci, ci, ci, vp, vp]),
             "xamgx_gen_batch_var5_nnz": (C.c_int64, [ci, vp]),
@@ -430,6 +432,12 @@ class Batch:
         check(lib().xamgx_batch_last_ms(self.id, C.byref(ms)))
         return ms.value
 
+    def launch(self):
+        """(workgroups, tiles per step, elements per SpMV lane, tuned) of the solver."""
+        v = [C.c_int() for _ in range(4)]
+        check(lib().xamgx_batch_get_launch(self.id, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     def destroy(self):
         check(lib().xamgx_batch_destroy(self.id))
 
@@ -459,6 +467,10 @@ def write_system(path, row, col, val, b=None, x=None):
 
 def batch_set_launch(grid=0, tile_rows=0, batch=0, use_graph=-1):
     check(lib().xamgx_batch_set_launch(grid, tile_rows, batch, use_graph))
+
+
+def batch_set_tuning(on, tps=0, ept=0):
+    check(lib().xamgx_batch_set_tuning(int(on), tps, ept))
 
 
 def batch_var5(nsys, seed, smin=16, smax=256, maxnv=4):
