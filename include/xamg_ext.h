@@ -68,7 +68,8 @@ int xamgx_bench_triad(uint64_t bytes, int iters, double *gbs);
 /* Average device time (ms) of one launch of an SpMV epilogue (0 y=Ax, 1 y+=Ax, 2 residual,
  * 3 Jacobi step, 4 Chebyshev step, 5 last Chebyshev step, 6 FP64 residual stored as FP32,
  * 7 and 8 Chebyshev step and last step on the scaled operator after the first step, 9 first
- * Chebyshev step on the scaled operator)
+ * Chebyshev step on the scaled operator, 10, 11 and 12 first, middle and last Chebyshev
+ * step on the unscaled operator storing their output multiplied by the row scaling)
  * with nd fused dot products (0, 1 or 3 as the solvers use them) on operand buffers of
  * the matrix's shape.  prec 0 FP64, 1 FP32, 2 BF16 values with FP32 vectors, 3 scaled
  * FP32.  cfg = {RHS per lane, workgroups, XCD mapping, LDS staging} or NULL for the

@@ -985,7 +985,7 @@ int xamgx_bench_spmv_epi(XAMG_matrix_id mid, int prec, int nv, int epi, int nd, 
         DevMatrix *A = slot(g_mats, mid, "matrix")->m.get();
         XAMG_REQUIRE(A != nullptr, "matrix not constructed");
         XAMG_REQUIRE(prec >= 0 && prec <= 3, "prec must be 0 (FP64), 1 (FP32), 2 (BF16) or 3 (scaled FP32)");
-        XAMG_REQUIRE(epi >= EPI_AXY && epi <= EPI_CHEB_SF, "unknown epilogue");
+        XAMG_REQUIRE(epi >= EPI_AXY && epi <= EPI_CHEB_LAST_U, "unknown epilogue");
         XAMG_REQUIRE(epi != EPI_RESID_MX || prec == 0, "the FP32-output residual needs FP64");
         XAMG_REQUIRE(iters > 0, "iters must be positive");
         const Prec vp = prec == 0 ? Prec::F64 : prec == 1 ? Prec::F32 : prec == 2 ? Prec::BF16 : Prec::F32S;

@@ -32,15 +32,32 @@ enum Epi : int {
     EPI_CHEB_LAST_S = 8, // as EPI_CHEB_S then y = 1*(y*d_i) [+ 1*xold]
     // first Horner step on the scaled operator (not also the last): x is the right-hand
     // side b, gathered as c_{k-1} (b_j d_j), then as EPI_CHEB_S
-    EPI_CHEB_SF = 9
+    EPI_CHEB_SF = 9,
+    // Horner steps on the unscaled operator whose output is the next step's input already
+    // multiplied by d (the scal that opens the next step, or the final one, chebyshev.inl:
+    // 172-180, done per row in the epilogue): the next step gathers x as it is, with no
+    // per-nonzero gather of the column scaling.  fl(t1_j * d_j) is the same product
+    // whichever kernel forms it, so the iterates are bit-identical to the reference's.
+    EPI_CHEB_FU = 10,    // first step: x = (c_{k-1}*(b_j*d_j))*d_j gathered from b;
+                         // y = 1*((c*(b*d_i) + 1*(acc*d_i))*d_i) [+ 1*xold]
+    EPI_CHEB_U = 11,     // middle steps, x pre-scaled: y as EPI_CHEB_FU
+    EPI_CHEB_LAST_U = 12 // last step (its own launch class), x pre-scaled: y as EPI_CHEB_FU
 };
 
 // Epilogue families of the Chebyshev steps.
 constexpr bool epi_cheb(int e) {
     return e == EPI_CHEB || e == EPI_CHEB_LAST || e == EPI_CHEB_S || e == EPI_CHEB_LAST_S ||
-           e == EPI_CHEB_SF;
+           e == EPI_CHEB_SF || e == EPI_CHEB_FU || e == EPI_CHEB_U || e == EPI_CHEB_LAST_U;
 }
-constexpr bool epi_cheb_last(int e) { return e == EPI_CHEB_LAST || e == EPI_CHEB_LAST_S; }
+// epilogues that end with y = 1*(y*d_i) [+ 1*xold]
+constexpr bool epi_cheb_last(int e) {
+    return e == EPI_CHEB_LAST || e == EPI_CHEB_LAST_S || e == EPI_CHEB_FU || e == EPI_CHEB_U ||
+           e == EPI_CHEB_LAST_U;
+}
+// unscaled-operator steps with compile-time row scaling t2 = acc*d_i
+constexpr bool epi_cheb_unscaled(int e) {
+    return e == EPI_CHEB_FU || e == EPI_CHEB_U || e == EPI_CHEB_LAST_U;
+}
 constexpr bool epi_cheb_scaled(int e) {
     return e == EPI_CHEB_S || e == EPI_CHEB_LAST_S || e == EPI_CHEB_SF;
 }

@@ -333,9 +333,11 @@ static const char *prec_tag(Prec p) {
 }
 
 static const char *epi_tag(int epi) {
-    static const char *t[] = {"axy",      "axpy",     "resid",  "jacobi",      "cheb",
-                              "cheb_last", "resid_mx", "cheb_s", "cheb_last_s", "cheb_sf"};
-    return epi >= 0 && epi < 10 ? t[epi] : "?";
+    static const char *t[] = {"axy",         "axpy",    "resid",   "jacobi",
+                              "cheb",        "cheb_last", "resid_mx", "cheb_s",
+                              "cheb_last_s", "cheb_sf",  "cheb_fu", "cheb_u",
+                              "cheb_last_u"};
+    return epi >= 0 && epi < 13 ? t[epi] : "?";
 }
 
 // Algorithmic HBM bytes of one SpMV launch: row pointers, column indices and values

@@ -542,7 +542,11 @@ class Chebyshev : public Solver<T> {
         } else {
             // The first Horner step gathers the right-hand side itself and forms
             // t1 = c_{k-1} (b d) scaled by d on the fly, so no separate start pass.
+            // On the unscaled operator every step stores its t1 already multiplied by d
+            // (the scal that opens the next step, chebyshev.inl:172), so the next step
+            // gathers it as it is; on the scaled operator the steps gather x as it is.
             T *t1 = this->vec(1), *t2 = this->vec(2);
+            const bool scaled = hprec == Prec::F32S;
             for (int q = order - 2; q >= 0; --q) {
                 SpmvArgs<T> s{};
                 const bool first = (q == order - 2);
@@ -553,21 +557,20 @@ class Chebyshev : public Solver<T> {
                 s.d = d;
                 s.dcol = d;
                 s.coef = c[q];
-                // steps on the scaled operator have compile-time epilogues: the first
-                // (when not also the last) gathers b scaled by d, the others x as it is
-                const bool lean = !first && hprec == Prec::F32S;
-                const bool lean_first = first && hprec == Prec::F32S;
+                int epi;
                 if (q > 0) {
                     s.y = t2;
-                    const int epi = lean ? EPI_CHEB_S : lean_first ? EPI_CHEB_SF : EPI_CHEB;
-                    spmv<T>(*A, hprec, epi, 0, nv, s, red_args(H), FinArgs<T>{});
-                    std::swap(t1, t2);
+                    epi = scaled ? (first ? EPI_CHEB_SF : EPI_CHEB_S)
+                                 : (first ? EPI_CHEB_FU : EPI_CHEB_U);
                 } else {
                     s.y = x;
                     s.xin = xzero ? nullptr : x;
-                    spmv<T>(*A, hprec, lean ? EPI_CHEB_LAST_S : EPI_CHEB_LAST, 0, nv, s,
-                            red_args(H), FinArgs<T>{});
+                    epi = scaled ? (first ? EPI_CHEB_LAST : EPI_CHEB_LAST_S)
+                                 : (first ? EPI_CHEB_FU : EPI_CHEB_LAST_U);
                 }
+                spmv<T>(*A, hprec, epi, 0, nv, s, red_args(H), FinArgs<T>{});
+                if (q > 0)
+                    std::swap(t1, t2);
             }
         }
         if (this->conv_check) {

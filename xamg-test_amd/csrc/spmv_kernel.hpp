@@ -203,6 +203,13 @@ __device__ __forceinline__ void x_transform(const SpmvArgs<T> &a, uint32_t c, T 
 #pragma unroll
         for (int v = 0; v < VPT; v++)
             xv[v] = a.xcoef * (xv[v] * dc);
+    } else if constexpr (EPI == EPI_CHEB_FU) {
+        // first Horner step read straight from the right-hand side:
+        // t1 = c_{k-1} * (b * d), then t1 * d (chebyshev.inl:169-173)
+        const T dc = a.dcol[c];
+#pragma unroll
+        for (int v = 0; v < VPT; v++)
+            xv[v] = (a.xcoef * (xv[v] * dc)) * dc;
     } else if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_LAST) {
         if (a.x_is_rhs && a.prescaled) {
             // first Horner step on the scaled operator: t1 = c_{k-1} (b d)
@@ -252,7 +259,9 @@ __device__ __forceinline__ void row_finish(const SpmvArgs<T> &a, int64_t e, int 
         for (int v = 0; v < VPT; v++) {
             const T rr = o.pb[v] * di;
             T t2 = acc[v];
-            if constexpr (!epi_cheb_scaled(EPI))
+            if constexpr (epi_cheb_unscaled(EPI))
+                t2 = acc[v] * di;
+            else if constexpr (!epi_cheb_scaled(EPI))
                 t2 = a.prescaled ? acc[v] : acc[v] * di;
             out[v] = a.coef * rr + (T)1.0 * t2;
         }
@@ -627,6 +636,9 @@ void spmv_nvv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r, const Fin
     XAMG_SPMV_CASE(EPI_CHEB_S, 0)
     XAMG_SPMV_CASE(EPI_CHEB_LAST_S, 0)
     XAMG_SPMV_CASE(EPI_CHEB_SF, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_FU, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_U, 0)
+    XAMG_SPMV_CASE(EPI_CHEB_LAST_U, 0)
     if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value) {
         XAMG_SPMV_CASE(EPI_RESID_MX, 0)
         XAMG_SPMV_CASE(EPI_RESID_MX, 1)

@@ -196,6 +196,9 @@ static void spmv_nzp_nv(int epi, int nd, const SpmvArgs<T> &a, const RedArgs &r,
     XAMG_NZP_CASE(EPI_CHEB_S, 0)
     XAMG_NZP_CASE(EPI_CHEB_LAST_S, 0)
     XAMG_NZP_CASE(EPI_CHEB_SF, 0)
+    XAMG_NZP_CASE(EPI_CHEB_FU, 0)
+    XAMG_NZP_CASE(EPI_CHEB_U, 0)
+    XAMG_NZP_CASE(EPI_CHEB_LAST_U, 0)
     if constexpr (std::is_same<T, double>::value && std::is_same<V, double>::value) {
         XAMG_NZP_CASE(EPI_RESID_MX, 0)
         XAMG_NZP_CASE(EPI_RESID_MX, 1)
