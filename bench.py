@@ -260,6 +260,18 @@ def _wl_cfg2(nv):
     return run
 
 
+def wl_hsgs128(X, tol, only_case, reps, warm, cpu):
+    """7-point 128^3, nv=8, PBiCGStab + HSGS (one symmetric sweep, the default n/128 core
+    blocks of 136 rows, each relaxed in LDS)."""
+    row, col, val = X.poisson7(128)
+    gs = {"method": "HSGS"}
+    rec = _run_cases(X, row, col, val, 8, 2,
+                     [("fp64", _fp64("PBiCGStab", gs, tol)),
+                      ("fp32_continued", _mixed("PBiCGStab", gs, tol))],
+                     only_case, reps, warm)
+    return "7pt-poisson 128^3 nv=8 pbicgstab+hsgs", rec
+
+
 def wl_aniso27(X, tol, only_case, reps, warm, cpu):
     """27-point anisotropic 160^3, nv=8: FP32 values (and BF16 values for the smoother)."""
     row, col, val = X.aniso27(160, 160, 160, seed=3)
@@ -348,6 +360,7 @@ WORKLOADS = {
     "cfg2-nv1": _wl_cfg2(1),
     "cfg2-nv4": _wl_cfg2(4),
     "cfg2-nv16": _wl_cfg2(16),
+    "hsgs128": wl_hsgs128,
     "aniso27": wl_aniso27,
     "amg192": wl_amg192,
     "batch": wl_batch,

@@ -700,37 +700,58 @@ __global__ void __launch_bounds__(kBlock)
     y[e] = (T)s;
 }
 
-// Block-hybrid SGS.  One workgroup per core block; the NV lanes of a row relax the
-// RHS in parallel while the rows of the block are visited in order (forward then
-// backward), exactly the sequential loop of csr_gs.inl:77-130 per block.
+// Block-hybrid SGS (csr_gs.inl:32-134 with blas2.h:225-263): t = b - A_offd x_old for
+// every row, then forward and backward relaxation inside each core block, exactly the
+// sequential loop of csr_gs.inl:77-130 per block.
+//
+// Off-block residual: one lane per (row, RHS) element; a row's entries are loaded eight
+// at a time (all column indices and values, then all x), then the off-block ones are
+// added in ascending column order.
 template <typename V, typename T, int NV>
 __global__ void __launch_bounds__(kBlock)
     k_hsgs_offd(const uint32_t *__restrict__ row, const uint32_t *__restrict__ col,
                 const V *__restrict__ val, int64_t n, const T *__restrict__ b,
-                const T *__restrict__ x, T *__restrict__ t, int nblocks, int64_t bsize,
-                const int *halt) {
+                const T *__restrict__ x, T *__restrict__ t, int64_t bsize, const int *halt) {
     if (halt != nullptr && *halt)
         return;
-    // t = 1*b + (-1)*(sum over off-block entries of x*a) for every row, grid-stride
+    constexpr int U = 8;
     const int64_t tot = n * NV;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = e / NV;
         const int l = (int)(e % NV);
-        const int64_t blk = i / bsize;
-        const int64_t off = blk * bsize;
+        const int64_t off = (i / bsize) * bsize;
         const int64_t end = off + bsize < n ? off + bsize : n;
+        const uint32_t j0 = row[i], j1 = row[i + 1];
+        const T bv = b[e];
         T acc = 0;
-        for (uint32_t j = row[i]; j < row[i + 1]; j++) {
-            const int64_t c = col[j];
-            if (c >= off && c < end)
-                continue;
-            acc = acc + x[c * NV + l] * to_t<T>(val[j]);
+        for (uint32_t j = j0; j < j1; j += U) {
+            uint32_t c[U];
+            T v[U], xv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool ok = j + u < j1;
+                // a padding slot reads the row's own (in-block) entry and is skipped
+                c[u] = ok ? col[j + u] : (uint32_t)i;
+                v[u] = ok ? to_t<T>(val[j + u]) : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                xv[u] = x[(int64_t)c[u] * NV + l];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t cc = c[u];
+                if (j + u < j1 && (cc < off || cc >= end))
+                    acc = acc + xv[u] * v[u];
+            }
         }
-        t[e] = (T)1.0 * b[e] + (T)(-1.0) * acc;
+        t[e] = (T)1.0 * bv + (T)(-1.0) * acc;
     }
 }
 
+// In-block sweep, global memory: one wavefront per block, the NV lanes of a row relax
+// the RHS in parallel while the rows are visited in order.  Kept for blocks whose copies
+// do not fit the LDS budget of k_hsgs_sweep.
 template <typename V, typename T, int NV>
 __global__ void __launch_bounds__(64)
     k_hsgs_diag(const uint32_t *__restrict__ row, const uint32_t *__restrict__ col,
@@ -761,6 +782,110 @@ __global__ void __launch_bounds__(64)
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Bytes of LDS k_hsgs_sweep takes for one block: the x and t slices, the inverse
+// diagonal, and the in-block entries as rs slots per row (values, 16-bit local columns)
+// with their counts.
+template <typename T>
+__host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) {
+    size_t b = sizeof(T) * (size_t)(2 * bsize * nv + bsize + bsize * rs);
+    b += 2 * (size_t)(bsize * rs) + 4 * (size_t)bsize;
+    return (b + 15) & ~(size_t)15;
+}
+
+// In-block sweep from LDS: one wavefront per block.  The wave copies the block's x and
+// t slices, inverse diagonal and in-block row slots (HsgsPlan) to LDS with consecutive
+// lanes on consecutive elements.  The NV lanes then run the forward and backward
+// relaxation: the next row's slots, t and diagonal are read (at addresses that follow
+// from the row index alone) while the current row is relaxed, so each row waits for one
+// round of LDS reads, its x entries.  The operations are those of the global-memory loop
+// in the same order, so the results are bit-identical.
+template <typename V, typename T, int NV, int RM>
+__global__ void __launch_bounds__(64)
+    k_hsgs_sweep(const uint16_t *__restrict__ gec, const T *__restrict__ gev,
+                 const int *__restrict__ grc, const T *__restrict__ inv_diag, int64_t n,
+                 const T *__restrict__ t, T *__restrict__ x, const T *__restrict__ relax,
+                 const T *__restrict__ iconv, int64_t bsize, const int *halt) {
+    extern __shared__ __align__(16) unsigned char hsgs_smem[];
+    if (halt != nullptr && *halt)
+        return;
+    const int64_t off = (int64_t)blockIdx.x * bsize;
+    if (off >= n)
+        return;
+    const int len = (int)(off + bsize < n ? bsize : n - off);
+    const int lane = threadIdx.x;
+    T *xs = (T *)hsgs_smem;
+    T *ts = xs + bsize * NV;
+    T *dg = ts + bsize * NV;
+    T *ev = dg + bsize;                            // [bsize][RM] values
+    uint16_t *ec = (uint16_t *)(ev + bsize * RM);  // [bsize][RM] local columns
+    int *rc = (int *)(ec + bsize * RM);            // [bsize] in-block entries per row
+
+    const int64_t off_e = off * NV, off_s = off * RM;
+    for (int k = lane; k < len * NV; k += 64) {
+        xs[k] = x[off_e + k];
+        ts[k] = t[off_e + k];
+    }
+    for (int k = lane; k < len * RM; k += 64) {
+        ev[k] = gev[off_s + k];
+        ec[k] = gec[off_s + k];
+    }
+    for (int r = lane; r < len; r += 64) {
+        dg[r] = inv_diag[off + r];
+        rc[r] = grc[off + r];
+    }
+    __syncthreads();
+    if (lane < NV) {
+        const T w = iconv != nullptr ? relax[lane] * iconv[lane] : relax[lane];
+        struct Row {
+            int cnt;
+            uint16_t c[RM];
+            T v[RM], t, d;
+        };
+        auto fetch = [&](Row &r, int i) {
+            r.cnt = rc[i];
+#pragma unroll
+            for (int u = 0; u < RM; u++) {
+                r.c[u] = ec[i * RM + u];
+                r.v[u] = ev[i * RM + u];
+            }
+            r.t = ts[i * NV + lane];
+            r.d = dg[i];
+        };
+        // relax row i from its slots in cur: its x entries are read first, then the
+        // next row's slots into nxt (inext < 0: none), then the row sum
+        auto step = [&](const Row &cur, int i, Row &nxt, int inext) {
+            T xv[RM];
+#pragma unroll
+            for (int u = 0; u < RM; u++)
+                xv[u] = xs[(int)cur.c[u] * NV + lane];
+            const T xi = xs[i * NV + lane];
+            if (inext >= 0)
+                fetch(nxt, inext);
+            T sum = cur.t;
+#pragma unroll
+            for (int u = 0; u < RM; u++) {
+                if (u >= cur.cnt)
+                    break;
+                sum -= xv[u] * cur.v[u];
+            }
+            const T coeff = w * cur.d;
+            xs[i * NV + lane] = xi + coeff * sum;
+        };
+        // forward then backward: step ii relaxes row ii, then row 2 len - 1 - ii; the two
+        // slot sets alternate, so no registers are copied between steps
+        auto rowof = [&](int ii) { return ii < len ? ii : 2 * len - 1 - ii; };
+        Row ra, rb;
+        fetch(ra, 0);
+        for (int ii = 0; ii < 2 * len; ii += 2) {
+            step(ra, rowof(ii), rb, rowof(ii + 1));
+            step(rb, rowof(ii + 1), ra, ii + 2 < 2 * len ? rowof(ii + 2) : -1);
+        }
+    }
+    __syncthreads();
+    for (int k = lane; k < len * NV; k += 64)
+        x[off_e + k] = xs[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -872,49 +997,123 @@ void launch_dense_mv(const TA *at, int64_t n, const T *x, T *y, int nv, const in
     XAMG_HIP_CHECK(hipGetLastError());
 }
 
+HsgsPlan hsgs_plan(const uint32_t *hrow, const uint32_t *hcol, int64_t n, int nblocks) {
+    HsgsPlan p;
+    if (n <= 0)
+        return p;
+    // core block size from split_range (misc.h:47-56) with 8-byte elements
+    int64_t cs = n / std::max(nblocks, 1);
+    cs += 8 - (cs % 8);
+    p.bsize = cs;
+    p.nblocks = (int)((n + cs - 1) / cs);
+    for (int64_t off = 0; off < n; off += cs) {
+        const int64_t end = std::min(n, off + cs);
+        for (int64_t i = off; i < end; i++) {
+            int64_t in = 0;
+            for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++)
+                in += (int64_t)hcol[j] >= off && (int64_t)hcol[j] < end;
+            p.max_row_in = std::max(p.max_row_in, in);
+        }
+    }
+    p.rs = p.max_row_in <= 4 ? 4 : p.max_row_in <= kHsgsRow ? kHsgsRow : 0;
+    return p;
+}
+
+template <typename T>
+void hsgs_slots(const HsgsPlan &p, const uint32_t *hrow, const uint32_t *hcol,
+                const double *hval, int64_t n, std::vector<uint16_t> &ec, std::vector<T> &ev,
+                std::vector<int> &rc) {
+    const int RM = p.rs;
+    ec.assign((size_t)n * RM, 0);
+    ev.assign((size_t)n * RM, (T)0);
+    rc.assign((size_t)n, 0);
+    for (int64_t off = 0; off < n; off += p.bsize) {
+        const int64_t end = std::min(n, off + p.bsize);
+        for (int64_t i = off; i < end; i++) {
+            int q = 0;
+            for (uint32_t j = hrow[i]; j < hrow[i + 1]; j++) {
+                const int64_t c = hcol[j];
+                if (c < off || c >= end)
+                    continue;
+                if (q >= RM)
+                    throw std::runtime_error("xamg: HSGS row slots overflow");
+                ec[(size_t)i * RM + q] = (uint16_t)(c - off);
+                ev[(size_t)i * RM + q] = (T)hval[j];
+                q++;
+            }
+            rc[i] = q;
+        }
+    }
+}
+template void hsgs_slots<double>(const HsgsPlan &, const uint32_t *, const uint32_t *,
+                                 const double *, int64_t, std::vector<uint16_t> &,
+                                 std::vector<double> &, std::vector<int> &);
+template void hsgs_slots<float>(const HsgsPlan &, const uint32_t *, const uint32_t *,
+                                const double *, int64_t, std::vector<uint16_t> &,
+                                std::vector<float> &, std::vector<int> &);
+
 template <typename V, typename T, int NV>
-static void hsgs_nv(const uint32_t *row, const uint32_t *col, const V *val, const T *inv_diag,
-                    int64_t n, const T *b, T *x, T *t, const T *relax, const T *iconv,
-                    int nblocks, int64_t bsize, const int *halt, hipStream_t s) {
+static void hsgs_offd_nv(const uint32_t *row, const uint32_t *col, const V *val, int64_t n,
+                         const T *b, const T *x, T *t, const HsgsPlan &p, const int *halt,
+                         hipStream_t s) {
     int64_t tot = n * NV;
-    int g1 = (int)std::min<int64_t>((tot + 255) / 256, 4096);
-    hipLaunchKernelGGL((k_hsgs_offd<V, T, NV>), dim3(g1 > 0 ? g1 : 1), dim3(256), 0, s, row, col,
-                       val, n, b, x, t, nblocks, bsize, halt);
-    hipLaunchKernelGGL((k_hsgs_diag<V, T, NV>), dim3(nblocks), dim3(64), 0, s, row, col, val,
-                       inv_diag, n, t, x, relax, iconv, bsize, halt);
+    int g1 = (int)std::min<int64_t>((tot + 255) / 256, 16384);
+    XAMG_LAUNCH((k_hsgs_offd<V, T, NV>), dim3(g1 > 0 ? g1 : 1), dim3(256), 0, s, row, col, val, n,
+                b, x, t, p.bsize, halt);
+}
+
+template <typename V, typename T, int NV>
+static void hsgs_sweep_nv(const uint32_t *row, const uint32_t *col, const V *val,
+                          const T *inv_diag, int64_t n, T *x, const T *t, const T *relax,
+                          const T *iconv, const HsgsPlan &p, const int *halt, hipStream_t s) {
+    const size_t lds = hsgs_lds_bytes<T>(p.bsize, NV, p.rs);
+    if (hsgs_uses_lds(p, NV, sizeof(T)) && p.rs == 4)
+        XAMG_LAUNCH((k_hsgs_sweep<V, T, NV, 4>), dim3(p.nblocks), dim3(64), lds, s, p.ec,
+                    (const T *)p.ev, p.rc, inv_diag, n, t, x, relax, iconv, p.bsize, halt);
+    else if (hsgs_uses_lds(p, NV, sizeof(T)))
+        XAMG_LAUNCH((k_hsgs_sweep<V, T, NV, kHsgsRow>), dim3(p.nblocks), dim3(64), lds, s, p.ec,
+                    (const T *)p.ev, p.rc, inv_diag, n, t, x, relax, iconv, p.bsize, halt);
+    else
+        XAMG_LAUNCH((k_hsgs_diag<V, T, NV>), dim3(p.nblocks), dim3(64), 0, s, row, col, val,
+                    inv_diag, n, t, x, relax, iconv, p.bsize, halt);
+}
+
+bool hsgs_uses_lds(const HsgsPlan &p, int nv, int es) {
+    const size_t lds = es == 8 ? hsgs_lds_bytes<double>(p.bsize, nv, p.rs)
+                               : hsgs_lds_bytes<float>(p.bsize, nv, p.rs);
+    return p.ec != nullptr && p.rs > 0 && lds <= kHsgsLdsMax;
+}
+
+#define XAMG_HSGS_NV(F, ...)                                                                       \
+    switch (nv) {                                                                                  \
+    case 1: F<V, T, 1>(__VA_ARGS__); break;                                                        \
+    case 2: F<V, T, 2>(__VA_ARGS__); break;                                                        \
+    case 4: F<V, T, 4>(__VA_ARGS__); break;                                                        \
+    case 8: F<V, T, 8>(__VA_ARGS__); break;                                                        \
+    case 16: F<V, T, 16>(__VA_ARGS__); break;                                                      \
+    default: throw std::runtime_error("xamg: nv must be one of 1, 2, 4, 8, 16");                    \
+    }
+
+template <typename V, typename T>
+void launch_hsgs_offd(const uint32_t *row, const uint32_t *col, const V *val, int64_t n, int nv,
+                      const T *b, const T *x, T *t, const HsgsPlan &p, const int *halt,
+                      hipStream_t s) {
+    if (n <= 0)
+        return;
+    XAMG_HSGS_NV(hsgs_offd_nv, row, col, val, n, b, x, t, p, halt, s)
+    XAMG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename V, typename T>
-void launch_hsgs(const uint32_t *row, const uint32_t *col, const V *val, const T *inv_diag,
-                 int64_t n, int nv, const T *b, T *x, T *t, const T *relax, const T *iconv,
-                 int nblocks, const int *halt, hipStream_t s) {
+void launch_hsgs_sweep(const uint32_t *row, const uint32_t *col, const V *val,
+                       const T *inv_diag, int64_t n, int nv, T *x, const T *t, const T *relax,
+                       const T *iconv, const HsgsPlan &p, const int *halt, hipStream_t s) {
     if (n <= 0)
         return;
-    // core block size from split_range (misc.h:47-56) with 8-byte elements
-    int64_t cs = n / nblocks;
-    cs += 8 - (cs % 8);
-    int nb = (int)((n + cs - 1) / cs);
-    switch (nv) {
-    case 1:
-        hsgs_nv<V, T, 1>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
-        break;
-    case 2:
-        hsgs_nv<V, T, 2>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
-        break;
-    case 4:
-        hsgs_nv<V, T, 4>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
-        break;
-    case 8:
-        hsgs_nv<V, T, 8>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
-        break;
-    case 16:
-        hsgs_nv<V, T, 16>(row, col, val, inv_diag, n, b, x, t, relax, iconv, nb, cs, halt, s);
-        break;
-    default:
-        throw std::runtime_error("xamg: nv must be one of 1, 2, 4, 8, 16");
-    }
+    XAMG_HSGS_NV(hsgs_sweep_nv, row, col, val, inv_diag, n, x, t, relax, iconv, p, halt, s)
     XAMG_HIP_CHECK(hipGetLastError());
 }
+#undef XAMG_HSGS_NV
 
 template <typename T>
 __global__ void k_state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap) {
@@ -969,14 +1168,16 @@ template void launch_dense_mv<float, float>(const float *, int64_t, const float 
                                             const int *, hipStream_t);
 template void launch_dense_mv<double, float>(const double *, int64_t, const float *, float *, int,
                                              const int *, hipStream_t);
-template void launch_hsgs<double, double>(const uint32_t *, const uint32_t *, const double *,
-                                          const double *, int64_t, int, const double *, double *,
-                                          double *, const double *, const double *, int,
-                                          const int *, hipStream_t);
-template void launch_hsgs<float, float>(const uint32_t *, const uint32_t *, const float *,
-                                        const float *, int64_t, int, const float *, float *,
-                                        float *, const float *, const float *, int, const int *,
-                                        hipStream_t);
+#define XAMG_HSGS_INST(T)                                                                          \
+    template void launch_hsgs_offd<T, T>(const uint32_t *, const uint32_t *, const T *, int64_t,     \
+                                         int, const T *, const T *, T *, const HsgsPlan &,          \
+                                         const int *, hipStream_t);                                 \
+    template void launch_hsgs_sweep<T, T>(const uint32_t *, const uint32_t *, const T *, const T *, \
+                                          int64_t, int, T *, const T *, const T *, const T *,       \
+                                          const HsgsPlan &, const int *, hipStream_t);
+XAMG_HSGS_INST(double)
+XAMG_HSGS_INST(float)
+#undef XAMG_HSGS_INST
 template void launch_state_init<double>(KrylovScalars<double> *, ConvState *, int, int,
                                         hipStream_t);
 template void launch_state_init<float>(KrylovScalars<float> *, ConvState *, int, int,

@@ -14,6 +14,8 @@
 
 #include "common.hpp"
 
+#include <vector>
+
 namespace xamg {
 
 // SpMV epilogues.  acc = sum_j x[col_j] * a_ij in ascending j (csr_matvec.inl:213-231).
@@ -283,13 +285,45 @@ template <typename TA, typename T>
 void launch_dense_mv(const TA *at, int64_t n, const T *x, T *y, int nv, const int *halt,
                      hipStream_t s);
 
-// Block-hybrid symmetric Gauss-Seidel sweep (csr_gs.inl:32-134 + blas2.h:225-263):
-// nblocks contiguous row blocks; t = b - A_offd x_old, then forward and backward
-// relaxation inside each block.  xnew may alias nothing; x is read-only.
+// Core blocks of the block-hybrid Gauss-Seidel sweep of one matrix: the block size of
+// split_range (misc.h:47-56) for nblocks blocks, the number of blocks, and the largest
+// number of in-block entries of a row.  Built on the host from the CSR.
+// When every row has at most kHsgsRow in-block entries, those entries are also kept
+// as rs slots per row in device memory (rs = 4 when every row has at most 4, else
+// kHsgsRow; ec: column - block start, ev: value in the sweep's precision, unused slots
+// 0; rc: entries per row), which the LDS sweep copies per block; otherwise (rs = 0, or
+// when a block's copies exceed kHsgsLdsMax bytes) the sweep runs from global memory.
+struct HsgsPlan {
+    int64_t bsize = 0;
+    int nblocks = 0;
+    int64_t max_row_in = 0;
+    int rs = 0;
+    const uint16_t *ec = nullptr;
+    const void *ev = nullptr;
+    const int *rc = nullptr;
+};
+constexpr int kHsgsRow = 8;
+constexpr size_t kHsgsLdsMax = 64 * 1024;
+HsgsPlan hsgs_plan(const uint32_t *hrow, const uint32_t *hcol, int64_t n, int nblocks);
+// Host copies of the row slots of plan p (values from hval, rounded to T).
+template <typename T>
+void hsgs_slots(const HsgsPlan &p, const uint32_t *hrow, const uint32_t *hcol,
+                const double *hval, int64_t n, std::vector<uint16_t> &ec, std::vector<T> &ev,
+                std::vector<int> &rc);
+
+// Block-hybrid symmetric Gauss-Seidel sweep (csr_gs.inl:32-134 + blas2.h:225-263) in two
+// launches: t = b - A_offd x_old (the off-block residual, t is scratch of n * nv
+// elements), then forward and backward relaxation of x inside each block.
 template <typename V, typename T>
-void launch_hsgs(const uint32_t *row, const uint32_t *col, const V *val, const T *inv_diag,
-                 int64_t n, int nv, const T *b, T *x, T *t, const T *relax, const T *iconv,
-                 int nblocks, const int *halt, hipStream_t s);
+void launch_hsgs_offd(const uint32_t *row, const uint32_t *col, const V *val, int64_t n, int nv,
+                      const T *b, const T *x, T *t, const HsgsPlan &p, const int *halt,
+                      hipStream_t s);
+template <typename V, typename T>
+void launch_hsgs_sweep(const uint32_t *row, const uint32_t *col, const V *val,
+                       const T *inv_diag, int64_t n, int nv, T *x, const T *t, const T *relax,
+                       const T *iconv, const HsgsPlan &p, const int *halt, hipStream_t s);
+// true when the in-block sweep of plan p runs from LDS (row slots built, copies fit)
+bool hsgs_uses_lds(const HsgsPlan &p, int nv, int es);
 
 template <typename T>
 void launch_state_init(KrylovScalars<T> *ks, ConvState *cs, int nv, int hist_cap, hipStream_t s);

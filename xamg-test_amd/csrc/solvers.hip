@@ -587,6 +587,8 @@ template <typename T>
 class HSGS : public Solver<T> {
     DBuf relax;
     int nblocks = 1;
+    HsgsPlan plan;
+    DBuf s_ec, s_ev, s_rc; // in-block row slots of the LDS sweep
 
   public:
     void setup() override {
@@ -598,21 +600,61 @@ class HSGS : public Solver<T> {
             nblocks = (int)this->prm.get_int("gs_blocks");
         else
             nblocks = (int)std::max<int64_t>(1, this->A->n / 128);
+        auto *A = this->A;
+        plan = hsgs_plan(A->hrow.data(), A->hcol.data(), A->n, nblocks);
+        // the slots hold the values the sweep reads: FP64, or FP64 rounded to FP32
+        const bool plain = std::is_same<T, double>::value ? this->vprec == Prec::F64
+                                                           : this->vprec == Prec::F32;
+        if (plain && plan.rs > 0 && plan.bsize <= 65535 && A->n > 0) {
+            std::vector<uint16_t> ec;
+            std::vector<T> ev;
+            std::vector<int> rc;
+            hsgs_slots<T>(plan, A->hrow.data(), A->hcol.data(), A->hval.data(), A->n, ec, ev, rc);
+            s_ec.alloc(sizeof(uint16_t) * ec.size());
+            s_ev.alloc(sizeof(T) * ev.size());
+            s_rc.alloc(sizeof(int) * rc.size());
+            XAMG_HIP_CHECK(hipMemcpy(s_ec.p, ec.data(), sizeof(uint16_t) * ec.size(), hipMemcpyHostToDevice));
+            XAMG_HIP_CHECK(hipMemcpy(s_ev.p, ev.data(), sizeof(T) * ev.size(), hipMemcpyHostToDevice));
+            XAMG_HIP_CHECK(hipMemcpy(s_rc.p, rc.data(), sizeof(int) * rc.size(), hipMemcpyHostToDevice));
+            plan.ec = s_ec.as<uint16_t>();
+            plan.ev = s_ev.p;
+            plan.rc = s_rc.as<int>();
+        }
     }
     void sweep(T *x, const T *b, const int *H, const T *mask) {
         auto *A = this->A;
         XAMG_REQUIRE(this->vprec != Prec::BF16, "HSGS supports FP64/FP32 values");
-        const void *val = A->values(this->vprec);
+        const T *val = (const T *)A->values(this->vprec);
+        const uint32_t *row = A->row.template as<uint32_t>(), *col = A->col.template as<uint32_t>();
+        const T *d = A->template inv_diag<T>();
+        const int nv = this->nv;
+        T *t = this->vec(1);
         hipStream_t s = Runtime::get().stream;
-        if constexpr (std::is_same<T, double>::value)
-            launch_hsgs<double, double>(A->row.template as<uint32_t>(), A->col.template as<uint32_t>(),
-                                        (const double *)val, A->template inv_diag<T>(), A->n,
-                                        this->nv, b, x, this->vec(1), relax.as<T>(), mask, nblocks,
-                                        H, s);
-        else
-            launch_hsgs<float, float>(A->row.template as<uint32_t>(), A->col.template as<uint32_t>(),
-                                      (const float *)val, A->template inv_diag<T>(), A->n, this->nv,
-                                      b, x, this->vec(1), relax.as<T>(), mask, nblocks, H, s);
+        KernelTimer &kt = KernelTimer::get();
+        const char *tp = sizeof(T) == 8 ? "f64" : "f32";
+        const double st = sizeof(T), n = (double)A->n;
+        const double E = n * nv * st, mat = 4.0 * (n + 1) + (double)A->nnz * (4 + st);
+        const bool lds = hsgs_uses_lds(plan, nv, (int)sizeof(T));
+        char cls[96];
+        int tok = -1;
+        if (kt.on()) {
+            // row pointers, indices and values, x (gathered once), b read, t written
+            snprintf(cls, sizeof(cls), "hsgs_offd %s nv%d n%lld", tp, nv, (long long)A->n);
+            tok = kt.begin(cls, mat + 3 * E);
+        }
+        launch_hsgs_offd<T, T>(row, col, val, A->n, nv, b, x, t, plan, H, s);
+        kt.end(tok);
+        tok = -1;
+        if (kt.on()) {
+            // x slice read and written, t and the inverse diagonal read, and the row slots
+            // (LDS sweep) or the rows' entries (global sweep)
+            snprintf(cls, sizeof(cls), "hsgs_sweep %s %s nv%d n%lld", tp, lds ? "lds" : "global",
+                     nv, (long long)A->n);
+            const double rows = lds ? n * plan.rs * (2 + st) + 4 * n : mat;
+            tok = kt.begin(cls, 3 * E + n * st + rows);
+        }
+        launch_hsgs_sweep<T, T>(row, col, val, d, A->n, nv, x, t, relax.as<T>(), mask, plan, H, s);
+        kt.end(tok);
     }
     void apply(T *x, bool xzero, const T *b, const int *halt) override {
         const int nv = this->nv;
