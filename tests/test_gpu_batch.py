@@ -153,6 +153,47 @@ def test_batch_mixed_refinement(xamg, small, op):
     assert (mx["checks"] >= 1).all()
 
 
+def test_batch_wide_columns(xamg):
+    """A system whose rows couple columns more than 32767 apart (a 5-point system of side
+    200 under a scattered symmetric permutation) beside two small ones: each system
+    within +-2 of the oracle's iteration range, error against a direct solve within 10x
+    the oracle's."""
+    rng = np.random.default_rng(21)
+    mats, nvs = [], []
+    for k, s in enumerate((200, 9, 17)):
+        row, col, val = X.var5(s, 30 + k)
+        n = s * s
+        A = sp.csr_matrix((val, col, row), shape=(n, n))
+        if k == 0:
+            perm = rng.permutation(n)
+            A = A[perm][:, perm].tocsr()
+            A.sort_indices()
+        mats.append(A)
+        nvs.append(1 + k)
+    Ab = sp.block_diag(mats, format="csr")
+    Ab.sort_indices()
+    sr = np.concatenate([[0], np.cumsum([m.shape[0] for m in mats])]).astype(np.int64)
+    nv = np.asarray(nvs, dtype=np.int64)
+    assert np.abs(Ab.indices - np.repeat(np.arange(Ab.shape[0]), np.diff(Ab.indptr))).max() > 32767
+    data = (None, nv, sr, Ab.indptr.astype(np.int64), Ab.indices.astype(np.int64), Ab.data)
+    off = _offsets(data)
+    b = X.rhs(int(off[-1]), 1, 23)
+    params = _params("PBiCGStab")
+    gpu = _run(data, params, b)
+    assert gpu["converged"].all() and gpu["relres"].max() <= 1.5e-10
+    for k, A in enumerate(mats):
+        n, nvk = A.shape[0], int(nv[k])
+        bk = b[off[k]:off[k + 1]]
+        # the batch sums each system's dots over 1024-row tiles: 40 of them here
+        runs = [O.solve(params, O.from_scipy(A), bk, nvk, nchunks=c) for c in CHUNKS + (40,)]
+        its = [r["iters"] for r in runs]
+        assert min(its) - 2 <= int(gpu["iters"][k]) <= max(its) + 2, (k, gpu["iters"][k], its)
+        xd = spl.spsolve(A.tocsc(), bk.reshape(n, nvk)).reshape(-1)
+        e_orc = max(np.linalg.norm(r["x"] - xd) for r in runs)
+        xg = gpu["x"][off[k]:off[k + 1]]
+        assert np.linalg.norm(xg - xd) <= 10 * e_orc + 1e-12 * np.linalg.norm(xd), k
+
+
 def test_batch_max_iters_and_zero_rhs(xamg, small):
     data, b = small
     b = b.copy()
