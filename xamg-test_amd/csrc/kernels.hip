@@ -784,29 +784,32 @@ __global__ void __launch_bounds__(64)
     }
 }
 
-// Bytes of LDS k_hsgs_sweep takes for one block: the x and t slices, the inverse
-// diagonal, and the in-block entries as rs slots per row (values, 16-bit local columns)
-// with their counts.
+// Bytes of LDS k_hsgs_sweep takes for one block: the x slice, the inverse diagonal, and
+// the in-block entries as rs slots per row (values, 16-bit local columns) with their
+// counts.
 template <typename T>
 __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) {
-    size_t b = sizeof(T) * (size_t)(2 * bsize * nv + bsize + bsize * rs);
+    size_t b = sizeof(T) * (size_t)(bsize * nv + bsize + bsize * rs);
     b += 2 * (size_t)(bsize * rs) + 4 * (size_t)bsize;
     return (b + 15) & ~(size_t)15;
 }
 
-// In-block sweep from LDS: one wavefront per block.  The wave copies the block's x and
-// t slices, inverse diagonal and in-block row slots (HsgsPlan) to LDS with consecutive
-// lanes on consecutive elements.  The NV lanes then run the forward and backward
-// relaxation: the next row's slots, t and diagonal are read (at addresses that follow
-// from the row index alone) while the current row is relaxed, so each row waits for one
-// round of LDS reads, its x entries.  The operations are those of the global-memory loop
-// in the same order, so the results are bit-identical.
+// In-block sweep from LDS: one wavefront per block.  The wave copies the block's x slice,
+// inverse diagonal and in-block row slots (HsgsPlan) to LDS with consecutive lanes on
+// consecutive elements.  The NV lanes then run the forward and backward relaxation: the
+// next row's slots and diagonal are read from LDS (at addresses that follow from the row
+// index alone) while the current row is relaxed, so each row waits for one round of LDS
+// reads, its x entries; each lane's t values come from global memory kHsgsTRing rows
+// ahead through a ring of registers (keeping them out of LDS lets more blocks share a
+// CU).  The operations are those of the global-memory loop in the same order, so the
+// results are bit-identical.
 template <typename V, typename T, int NV, int RM>
 __global__ void __launch_bounds__(64)
     k_hsgs_sweep(const uint16_t *__restrict__ gec, const T *__restrict__ gev,
                  const int *__restrict__ grc, const T *__restrict__ inv_diag, int64_t n,
                  const T *__restrict__ t, T *__restrict__ x, const T *__restrict__ relax,
                  const T *__restrict__ iconv, int64_t bsize, const int *halt) {
+    constexpr int P = kHsgsTRing;
     extern __shared__ __align__(16) unsigned char hsgs_smem[];
     if (halt != nullptr && *halt)
         return;
@@ -816,17 +819,14 @@ __global__ void __launch_bounds__(64)
     const int len = (int)(off + bsize < n ? bsize : n - off);
     const int lane = threadIdx.x;
     T *xs = (T *)hsgs_smem;
-    T *ts = xs + bsize * NV;
-    T *dg = ts + bsize * NV;
+    T *dg = xs + bsize * NV;
     T *ev = dg + bsize;                            // [bsize][RM] values
     uint16_t *ec = (uint16_t *)(ev + bsize * RM);  // [bsize][RM] local columns
     int *rc = (int *)(ec + bsize * RM);            // [bsize] in-block entries per row
 
     const int64_t off_e = off * NV, off_s = off * RM;
-    for (int k = lane; k < len * NV; k += 64) {
+    for (int k = lane; k < len * NV; k += 64)
         xs[k] = x[off_e + k];
-        ts[k] = t[off_e + k];
-    }
     for (int k = lane; k < len * RM; k += 64) {
         ev[k] = gev[off_s + k];
         ec[k] = gec[off_s + k];
@@ -841,7 +841,7 @@ __global__ void __launch_bounds__(64)
         struct Row {
             int cnt;
             uint16_t c[RM];
-            T v[RM], t, d;
+            T v[RM], d;
         };
         auto fetch = [&](Row &r, int i) {
             r.cnt = rc[i];
@@ -850,12 +850,11 @@ __global__ void __launch_bounds__(64)
                 r.c[u] = ec[i * RM + u];
                 r.v[u] = ev[i * RM + u];
             }
-            r.t = ts[i * NV + lane];
             r.d = dg[i];
         };
-        // relax row i from its slots in cur: its x entries are read first, then the
-        // next row's slots into nxt (inext < 0: none), then the row sum
-        auto step = [&](const Row &cur, int i, Row &nxt, int inext) {
+        // relax row i (its t value tv) from its slots in cur: its x entries are read
+        // first, then the next row's slots into nxt (inext < 0: none), then the row sum
+        auto step = [&](const Row &cur, int i, T tv, Row &nxt, int inext) {
             T xv[RM];
 #pragma unroll
             for (int u = 0; u < RM; u++)
@@ -863,7 +862,7 @@ __global__ void __launch_bounds__(64)
             const T xi = xs[i * NV + lane];
             if (inext >= 0)
                 fetch(nxt, inext);
-            T sum = cur.t;
+            T sum = tv;
 #pragma unroll
             for (int u = 0; u < RM; u++) {
                 if (u >= cur.cnt)
@@ -873,14 +872,41 @@ __global__ void __launch_bounds__(64)
             const T coeff = w * cur.d;
             xs[i * NV + lane] = xi + coeff * sum;
         };
-        // forward then backward: step ii relaxes row ii, then row 2 len - 1 - ii; the two
-        // slot sets alternate, so no registers are copied between steps
-        auto rowof = [&](int ii) { return ii < len ? ii : 2 * len - 1 - ii; };
+        // forward then backward: step s relaxes row s, then row 2 len - 1 - s; the two
+        // slot sets alternate (P is even), so no registers are copied between steps
+        const int ns = 2 * len;
+        auto rowof = [&](int s) { return s < len ? s : ns - 1 - s; };
+        // the ring's loads take the row of step min(s + P, ns - 1), so they are issued
+        // without a branch; full rounds of P steps, then the remaining steps
+        auto tload = [&](int s) { return t[(off + rowof(s < ns ? s : ns - 1)) * NV + lane]; };
+        T ring[P];
+#pragma unroll
+        for (int k = 0; k < P; k++)
+            ring[k] = tload(k);
         Row ra, rb;
         fetch(ra, 0);
-        for (int ii = 0; ii < 2 * len; ii += 2) {
-            step(ra, rowof(ii), rb, rowof(ii + 1));
-            step(rb, rowof(ii + 1), ra, ii + 2 < 2 * len ? rowof(ii + 2) : -1);
+        int s0 = 0;
+        for (; s0 + P <= ns; s0 += P) {
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                const int s = s0 + k;
+                const T tv = ring[k];
+                ring[k] = tload(s + P);
+                const int inext = s + 1 < ns ? rowof(s + 1) : -1;
+                if (k & 1)
+                    step(rb, rowof(s), tv, ra, inext);
+                else
+                    step(ra, rowof(s), tv, rb, inext);
+            }
+        }
+        // fewer than P steps remain (an even number: ns and P are even)
+#pragma unroll
+        for (int k = 0; k < P; k += 2) {
+            const int s = s0 + k;
+            if (s >= ns)
+                break;
+            step(ra, rowof(s), ring[k], rb, rowof(s + 1));
+            step(rb, rowof(s + 1), ring[k + 1], ra, s + 2 < ns ? rowof(s + 2) : -1);
         }
     }
     __syncthreads();

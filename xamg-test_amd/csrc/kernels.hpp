@@ -304,6 +304,8 @@ struct HsgsPlan {
 };
 constexpr int kHsgsRow = 8;
 constexpr size_t kHsgsLdsMax = 64 * 1024;
+// rows ahead of the relaxation at which the LDS sweep loads a lane's t values (even)
+constexpr int kHsgsTRing = 16;
 HsgsPlan hsgs_plan(const uint32_t *hrow, const uint32_t *hcol, int64_t n, int nblocks);
 // Host copies of the row slots of plan p (values from hval, rounded to T).
 template <typename T>
