@@ -232,7 +232,8 @@ def wl_small32(X, tol, only_case, reps, warm, cpu):
     complete oracle solve of the same system on the host cores beside it."""
     row, col, val = X.poisson7(32)
     p = _fp64("PBiCGStab", {"method": "Jacobi"}, 1e-8)
-    rec = _run_cases(X, row, col, val, 1, 1, [("fp64", p)], only_case, reps, warm)
+    rec = _run_cases(X, row, col, val, 1, 1, [("fp64", p)], only_case, reps, warm,
+                     tight=_fp64("PBiCGStab", {"method": "Jacobi"}, TIGHT_TOL))
     if cpu:
         import oracle as O
         b = X.rhs(32 ** 3, 1, 1)
@@ -255,7 +256,8 @@ def _wl_cfg2(nv):
         rec = _run_cases(X, row, col, val, nv, 2,
                          [("fp64", _fp64("PBiCGStab", cheb, tol)),
                           ("fp32_continued", _mixed("PBiCGStab", cheb, tol))],
-                         only_case, reps, warm, eigs=(1 - c, 1 + c))
+                         only_case, reps, warm, eigs=(1 - c, 1 + c),
+                         tight=_fp64("PBiCGStab", cheb, TIGHT_TOL))
         return f"7pt-poisson 128^3 nv={nv} pbicgstab+cheb4", rec
     return run
 
@@ -268,7 +270,7 @@ def wl_hsgs128(X, tol, only_case, reps, warm, cpu):
     rec = _run_cases(X, row, col, val, 8, 2,
                      [("fp64", _fp64("PBiCGStab", gs, tol)),
                       ("fp32_continued", _mixed("PBiCGStab", gs, tol))],
-                     only_case, reps, warm)
+                     only_case, reps, warm, tight=_fp64("PBiCGStab", gs, TIGHT_TOL))
     return "7pt-poisson 128^3 nv=8 pbicgstab+hsgs", rec
 
 
@@ -281,7 +283,7 @@ def wl_aniso27(X, tol, only_case, reps, warm, cpu):
                       ("fp32_continued", _mixed("PBiCGStab", jac, tol)),
                       ("fp32_continued_bf16_smoother",
                        _mixed("PBiCGStab", dict(jac, values_precision="bf16"), tol))],
-                     only_case, reps, warm)
+                     only_case, reps, warm, tight=_fp64("PBiCGStab", jac, TIGHT_TOL))
     return "27pt-aniso 160^3 nv=8 pbicgstab+jacobi", rec
 
 
@@ -309,9 +311,11 @@ def wl_amg192(X, tol, only_case, reps, warm, cpu):
            "pre_smoother": {"method": "Chebyshev"}, "post_smoother": {"method": "Chebyshev"},
            "coarse_grid_solver": {"method": "Direct", "fp64_solve": True}}
     c = np.cos(np.pi / 193)
+    mgt = json.loads(json.dumps(mg))
+    mgt["solver"]["rel_tolerance"] = TIGHT_TOL
     rec = _run_cases(X, row, col, val, 16, 4,
                      [("fp64_levels", mg), ("fp32_levels", mgr), ("fp32_continued_fp32_cycle", mgc)],
-                     only_case, reps, warm, eigs=(1 - c, 1 + c), hier=H)
+                     only_case, reps, warm, eigs=(1 - c, 1 + c), hier=H, tight=mgt)
     rec["host_setup_s"] = setup_s
     rec["levels"] = H.levels()
     H.destroy()
@@ -323,10 +327,22 @@ def wl_batch(X, tol, only_case, reps, warm, cpu):
     side, nvk, sr, row, col, val = X.batch_var5(2000, 5, smin=16, smax=256, maxnv=4)
     B = X.Batch(sr, nvk, row, col, val)
     x, bv = B.vector(), B.vector()
+    B_off = np.concatenate([[0], np.cumsum(np.diff(sr).astype(np.int64) * np.asarray(nvk))])
     b = X.rhs(B.nelem, 1, 5)
     bv.upload(b)
     rec = {}
     jac = {"method": "Jacobi"}
+    n_el = B.nelem
+    xt = None
+    if only_case is None:
+        # the FP64 batch solved to 1e-12 (as in the 2,000-system GPU test): the
+        # reference solution of the error figures
+        B.setup(_fp64("PBiCGStab", jac, 1e-12, max_iters=5000))
+        x.upload(np.zeros(n_el))
+        B.solve(x, bv)
+        xt = x.download()
+        rec["tight_fp64_rel_tolerance"] = 1e-12
+        rec["tight_fp64_relres_verified"] = float(B.residuals(x, bv).max())
     for label, params in (("fp64", _fp64("PBiCGStab", jac, tol, max_iters=5000)),
                           ("fp32_continued", _mixed("PBiCGStab", jac, tol, max_iters=5000))):
         if only_case is not None and label != only_case:
@@ -346,6 +362,16 @@ def wl_batch(X, tol, only_case, reps, warm, cpu):
                       "row_iterations": float(np.dot(it.astype(np.float64),
                                                      np.diff(sr).astype(np.float64))),
                       "converged": bool(conv.all()), "relres_verified": float(rr.max())}
+        if xt is not None:
+            # largest relative error over the systems' right-hand sides
+            xs = x.download()
+            err = 0.0
+            for k in range(len(nvk)):
+                a, e = int(B_off[k]), int(B_off[k + 1])
+                xk, tk = xs[a:e].reshape(-1, nvk[k]), xt[a:e].reshape(-1, nvk[k])
+                err = max(err, float((np.linalg.norm(xk - tk, axis=0)
+                                      / np.linalg.norm(tk, axis=0)).max()))
+            rec[label]["err_vs_tight_fp64"] = err
     if "fp64" in rec and "fp32_continued" in rec:
         rec["fp32_continued"]["speedup_vs_fp64"] = rec["fp64"]["s"] / rec["fp32_continued"]["s"]
     rec["rows"] = int(sr[-1])
