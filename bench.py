@@ -40,6 +40,9 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "xamg-test_amd")]
 
 HBM_PEAK_GBS = 8000.0
 TIGHT_TOL = 1e-13
+# host seconds of the oracle's FP64 solve beside each other single-system workload (the
+# iterations that fit are timed and scaled when the complete solve would not fit)
+CPU_BUDGET_OTHER = 10.0
 # launch classes kept per instrumented solve of the other workloads (all of them with --only)
 INSTRUMENT_TOP = 3
 
@@ -258,6 +261,10 @@ def _wl_cfg2(nv):
                           ("fp32_continued", _mixed("PBiCGStab", cheb, tol))],
                          only_case, reps, warm, eigs=(1 - c, 1 + c),
                          tight=_fp64("PBiCGStab", cheb, TIGHT_TOL))
+        if cpu and "fp64" in rec:
+            rec["cpu_baseline"] = cpu_baseline(row, col, val, X.rhs(128 ** 3, nv, 2), nv,
+                                               "cheb4", rec["fp64"]["iters"][0], (1 - c, 1 + c),
+                                               budget_s=CPU_BUDGET_OTHER)
         return f"7pt-poisson 128^3 nv={nv} pbicgstab+cheb4", rec
     return run
 
@@ -284,6 +291,10 @@ def wl_aniso27(X, tol, only_case, reps, warm, cpu):
                       ("fp32_continued_bf16_smoother",
                        _mixed("PBiCGStab", dict(jac, values_precision="bf16"), tol))],
                      only_case, reps, warm, tight=_fp64("PBiCGStab", jac, TIGHT_TOL))
+    if cpu and "fp64" in rec:
+        rec["cpu_baseline"] = cpu_baseline(row, col, val, X.rhs(160 ** 3, 8, 3), 8, "jacobi",
+                                           rec["fp64"]["iters"][0], (None, None),
+                                           budget_s=CPU_BUDGET_OTHER)
     return "27pt-aniso 160^3 nv=8 pbicgstab+jacobi", rec
 
 
