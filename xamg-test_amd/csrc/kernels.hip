@@ -786,11 +786,13 @@ __global__ void __launch_bounds__(64)
 
 // Bytes of LDS k_hsgs_sweep takes for one block: the x slice, the inverse diagonal, and
 // the in-block entries as rs slots per row (values, 16-bit local columns) with their
-// counts.
+// counts; in FP32 also the t slice.
 template <typename T>
 __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) {
     size_t b = sizeof(T) * (size_t)(bsize * nv + bsize + bsize * rs);
     b += 2 * (size_t)(bsize * rs) + 4 * (size_t)bsize;
+    if (sizeof(T) < 8)
+        b += sizeof(T) * (size_t)(bsize * nv); // FP32: the t slice too
     return (b + 15) & ~(size_t)15;
 }
 
@@ -799,10 +801,12 @@ __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) 
 // consecutive elements.  The NV lanes then run the forward and backward relaxation: the
 // next row's slots and diagonal are read from LDS (at addresses that follow from the row
 // index alone) while the current row is relaxed, so each row waits for one round of LDS
-// reads, its x entries; each lane's t values come from global memory kHsgsTRing rows
-// ahead through a ring of registers (keeping them out of LDS lets more blocks share a
-// CU).  The operations are those of the global-memory loop in the same order, so the
-// results are bit-identical.
+// reads, its x entries.  In FP64 each lane's t values come from global memory kHsgsTRing
+// rows ahead through a ring of registers (keeping t out of LDS lets more blocks share a
+// CU: 628 instead of 721 us per sweep at 128^3 nv=8); in FP32 the t slice is copied to LDS
+// too and read with the row's slots (the global ring made the FP32 sweep slower, 397
+// against 376 us).  The operations are those of the global-memory loop in the same order,
+// so the results are bit-identical.
 template <typename V, typename T, int NV, int RM>
 __global__ void __launch_bounds__(64)
     k_hsgs_sweep(const uint16_t *__restrict__ gec, const T *__restrict__ gev,
@@ -823,10 +827,15 @@ __global__ void __launch_bounds__(64)
     T *ev = dg + bsize;                            // [bsize][RM] values
     uint16_t *ec = (uint16_t *)(ev + bsize * RM);  // [bsize][RM] local columns
     int *rc = (int *)(ec + bsize * RM);            // [bsize] in-block entries per row
+    constexpr bool TG = sizeof(T) == 8;            // t from global memory
+    T *ts = (T *)(rc + bsize);                     // [bsize][NV] t slice (FP32)
 
     const int64_t off_e = off * NV, off_s = off * RM;
-    for (int k = lane; k < len * NV; k += 64)
+    for (int k = lane; k < len * NV; k += 64) {
         xs[k] = x[off_e + k];
+        if constexpr (!TG)
+            ts[k] = t[off_e + k];
+    }
     for (int k = lane; k < len * RM; k += 64) {
         ev[k] = gev[off_s + k];
         ec[k] = gec[off_s + k];
@@ -841,7 +850,7 @@ __global__ void __launch_bounds__(64)
         struct Row {
             int cnt;
             uint16_t c[RM];
-            T v[RM], d;
+            T v[RM], d, t;
         };
         auto fetch = [&](Row &r, int i) {
             r.cnt = rc[i];
@@ -851,6 +860,8 @@ __global__ void __launch_bounds__(64)
                 r.v[u] = ev[i * RM + u];
             }
             r.d = dg[i];
+            if constexpr (!TG)
+                r.t = ts[i * NV + lane];
         };
         // relax row i (its t value tv) from its slots in cur: its x entries are read
         // first, then the next row's slots into nxt (inext < 0: none), then the row sum
@@ -876,37 +887,44 @@ __global__ void __launch_bounds__(64)
         // slot sets alternate (P is even), so no registers are copied between steps
         const int ns = 2 * len;
         auto rowof = [&](int s) { return s < len ? s : ns - 1 - s; };
-        // the ring's loads take the row of step min(s + P, ns - 1), so they are issued
-        // without a branch; full rounds of P steps, then the remaining steps
-        auto tload = [&](int s) { return t[(off + rowof(s < ns ? s : ns - 1)) * NV + lane]; };
-        T ring[P];
-#pragma unroll
-        for (int k = 0; k < P; k++)
-            ring[k] = tload(k);
         Row ra, rb;
         fetch(ra, 0);
-        int s0 = 0;
-        for (; s0 + P <= ns; s0 += P) {
+        if constexpr (TG) {
+            // the ring's loads take the row of step min(s + P, ns - 1), so they are
+            // issued without a branch; full rounds of P steps, then the remaining steps
+            auto tload = [&](int s) { return t[(off + rowof(s < ns ? s : ns - 1)) * NV + lane]; };
+            T ring[P];
 #pragma unroll
-            for (int k = 0; k < P; k++) {
-                const int s = s0 + k;
-                const T tv = ring[k];
-                ring[k] = tload(s + P);
-                const int inext = s + 1 < ns ? rowof(s + 1) : -1;
-                if (k & 1)
-                    step(rb, rowof(s), tv, ra, inext);
-                else
-                    step(ra, rowof(s), tv, rb, inext);
+            for (int k = 0; k < P; k++)
+                ring[k] = tload(k);
+            int s0 = 0;
+            for (; s0 + P <= ns; s0 += P) {
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    const int s = s0 + k;
+                    const T tv = ring[k];
+                    ring[k] = tload(s + P);
+                    const int inext = s + 1 < ns ? rowof(s + 1) : -1;
+                    if (k & 1)
+                        step(rb, rowof(s), tv, ra, inext);
+                    else
+                        step(ra, rowof(s), tv, rb, inext);
+                }
             }
-        }
-        // fewer than P steps remain (an even number: ns and P are even)
+            // fewer than P steps remain (an even number: ns and P are even)
 #pragma unroll
-        for (int k = 0; k < P; k += 2) {
-            const int s = s0 + k;
-            if (s >= ns)
-                break;
-            step(ra, rowof(s), ring[k], rb, rowof(s + 1));
-            step(rb, rowof(s + 1), ring[k + 1], ra, s + 2 < ns ? rowof(s + 2) : -1);
+            for (int k = 0; k < P; k += 2) {
+                const int s = s0 + k;
+                if (s >= ns)
+                    break;
+                step(ra, rowof(s), ring[k], rb, rowof(s + 1));
+                step(rb, rowof(s + 1), ring[k + 1], ra, s + 2 < ns ? rowof(s + 2) : -1);
+            }
+        } else {
+            for (int s = 0; s < ns; s += 2) {
+                step(ra, rowof(s), ra.t, rb, rowof(s + 1));
+                step(rb, rowof(s + 1), rb.t, ra, s + 2 < ns ? rowof(s + 2) : -1);
+            }
         }
     }
     __syncthreads();
