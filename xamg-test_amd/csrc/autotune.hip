@@ -7,8 +7,9 @@
 // itself.  Each candidate launch is timed cold: a 512 MiB copy before it evicts the L2
 // caches and the 256 MB last-level cache.  The fastest configuration of each (staging
 // mode, XCD mapping) pair is a finalist.  In solve: the first launches of the class inside
-// a solver rotate over the finalists with HIP events around each, and the one with the
-// smallest median time becomes the class's configuration (launch_choice): inside a solver
+// a solver rotate over the finalists with HIP events around each, the three with the
+// smallest median times are timed again (seven launches each in all), and the one with the
+// smallest median becomes the class's configuration (launch_choice): inside a solver
 // the previous kernels leave part of a launch's operands in the last-level cache, which
 // the cold timing cannot see.  Results are cached per process under a key of the matrix
 // shape and optionally in a text file (XAMG_TUNE_FILE) across processes.
@@ -304,6 +305,7 @@ const DevMatrix::Tune &DevMatrix::tuned(Prec vp, int es, int nv, int epi, int nd
         o.cold_ms = r.finalist_ms;
         o.got.assign(o.cand.size(), {});
         o.issued.assign(o.cand.size(), 0);
+        o.need.assign(o.cand.size(), kOnlineSamples);
     }
     return tune[slot];
 }
@@ -349,14 +351,30 @@ DevMatrix::Tune DevMatrix::launch_choice(Prec vp, int es, int nv, int epi, int n
     const int nc = (int)o.cand.size();
     bool complete = true;
     for (int c = 0; c < nc; c++)
-        complete = complete && (int)o.got[c].size() >= kOnlineSamples;
+        complete = complete && (int)o.got[c].size() >= o.need[c];
+    auto median = [&](int c) {
+        std::vector<float> v = o.got[c];
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    };
+    if (complete && !o.second && nc > kOnlineFinal) {
+        // first round done: the closest finalists get more samples, the others drop out
+        std::vector<int> ord(nc);
+        for (int c = 0; c < nc; c++)
+            ord[c] = c;
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return median(a) < median(b); });
+        for (int k = 0; k < nc; k++)
+            o.need[ord[k]] = k < kOnlineFinal ? kOnlineSamples2 : 0;
+        o.second = true;
+        complete = false;
+    }
     if (complete) {
         int best = 0;
         float best_med = 1e30f;
         for (int c = 0; c < nc; c++) {
-            std::vector<float> v = o.got[c];
-            std::sort(v.begin(), v.end());
-            const float med = v[v.size() / 2];
+            if (o.need[c] == 0)
+                continue;
+            const float med = median(c);
             if (med < best_med) {
                 best_med = med;
                 best = c;
@@ -370,7 +388,7 @@ DevMatrix::Tune DevMatrix::launch_choice(Prec vp, int es, int nv, int epi, int n
     }
     for (int k = 0; k < nc; k++) {
         const int c = (o.next + k) % nc;
-        if (o.issued[c] < kOnlineSamples) {
+        if (o.issued[c] < o.need[c]) {
             o.next = (c + 1) % nc;
             o.issued[c]++;
             *trial = c;

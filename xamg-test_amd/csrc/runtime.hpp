@@ -116,9 +116,10 @@ struct DevMatrix {
 
     // In-solve choice among the finalists of the cold timing (the fastest configuration
     // of each staging mode and XCD mapping): the first launches of a class inside the
-    // solver rotate over the finalists with HIP events around each launch, and after
-    // kOnlineSamples timed launches of every finalist the one with the smallest median
-    // device time becomes the class's configuration.  A single launch timed after a
+    // solver rotate over the finalists with HIP events around each launch; after
+    // kOnlineSamples timed launches of every finalist the kOnlineFinal with the smallest
+    // median device times are timed again up to kOnlineSamples2 launches each, and the
+    // one with the smallest median becomes the class's configuration.  A single launch timed after a
     // cache flush misranks the modes: inside a solver the previous kernel leaves part of
     // this launch's operands in the last-level cache.  Every finalist computes the same
     // bits (row sums in column order; the dot kernels' finalists share the workgroup
@@ -133,13 +134,19 @@ struct DevMatrix {
         std::vector<float> cold_ms;           // cold time of each finalist
         std::vector<std::vector<float>> got;  // in-solve samples per finalist
         std::vector<int> issued;
+        std::vector<int> need; // samples still wanted per finalist (0: out of the running)
         std::vector<Sample> pending;
         int next = 0;
         int dropped = 0; // samples of launches that did almost nothing
+        bool second = false; // the second round (the closest finalists) has started
         bool done = false;
     };
     std::map<int, Online> online;
     static constexpr int kOnlineSamples = 3;
+    // second round: the kOnlineFinal finalists with the smallest medians get
+    // kOnlineSamples2 samples in all before the choice
+    static constexpr int kOnlineFinal = 3;
+    static constexpr int kOnlineSamples2 = 7;
     // configuration of the next launch of a class (a finalist while the in-solve choice
     // runs); *trial >= 0 (the finalist's index) asks for online_begin() right before and
     // online_end() right after the launch
