@@ -7,7 +7,8 @@ order, so the GPU iterate must equal the oracle's bit for bit.  The cases cover 
 kernel that stages a block in LDS (blocks whose copies fit 64 KiB and whose rows have
 at most 8 in-block entries: the default n/128 blocks of 136 rows) and the global-memory
 sweep (larger blocks, rows with more in-block entries), 1-16 right-hand sides, 7- and
-27-point stencils and a ragged random matrix with empty off-diagonal rows."""
+27-point stencils, a ragged random matrix with empty off-diagonal rows, and FP32 sweeps
+as the inner solver of an IterativeRefinement step."""
 import numpy as np
 import pytest
 
@@ -96,3 +97,27 @@ def test_hsgs_ragged_rows(xamg, blocks):
         row[i + 1] = len(col)
     _check(xamg, row, np.asarray(col, dtype=np.int64), np.asarray(val), 4, gs_blocks=blocks,
            gpu_blocks=blocks)
+
+
+@pytest.mark.parametrize("nv", [1, 8])
+def test_hsgs_fp32_inner_bit_exact(xamg, nv):
+    """FP32 HSGS sweeps (the LDS sweep with the t slice in LDS) as the inner solver of one
+    IterativeRefinement step from x0 = 0 (ir.inl:107-194): x = (double) of two FP32 sweeps
+    on fl32(b), bit for bit against the oracle's FP32 restatement."""
+    nx = 32
+    row, col, val = xamg.poisson7(nx)
+    n = nx ** 3
+    b = xamg.rhs(n, nv, 7)
+    params = {"meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
+                              "max_iters": 1, "rel_tolerance": 1e-30, "abs_tolerance": 0.0},
+              "solver": {"method": "HSGS", "max_iters": 2, "relax_factor": 0.9,
+                         "convergence_details": "nocheck"}}
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    orc = O.solve(dict(params, solver=dict(params["solver"], gs_blocks=n // 128)),
+                  O.csr(row, col, val), b, nv)
+    xg, xo = np.asarray(gpu["x"]), np.asarray(orc["x"])
+    bad = np.flatnonzero(xg.view(np.uint64) != xo.view(np.uint64))
+    assert bad.size == 0, (bad[:8], xg[bad[:4]], xo[bad[:4]])
+    # the result is an FP32 solve widened to FP64
+    assert np.array_equal(xg.astype(np.float32).astype(np.float64), xg)
+    assert np.abs(xg).max() > 0
