@@ -99,21 +99,25 @@ def test_hsgs_ragged_rows(xamg, blocks):
            gpu_blocks=blocks)
 
 
-@pytest.mark.parametrize("nv", [1, 8])
-def test_hsgs_fp32_inner_bit_exact(xamg, nv):
-    """FP32 HSGS sweeps (the LDS sweep with the t slice in LDS) as the inner solver of one
-    IterativeRefinement step from x0 = 0 (ir.inl:107-194): x = (double) of two FP32 sweeps
-    on fl32(b), bit for bit against the oracle's FP32 restatement."""
+@pytest.mark.parametrize("nv,blocks", [(1, None), (8, None), (4, 2)])
+def test_hsgs_fp32_inner_bit_exact(xamg, nv, blocks):
+    """FP32 HSGS sweeps as the inner solver of one IterativeRefinement step from x0 = 0
+    (ir.inl:107-194): x = (double) of two FP32 sweeps on fl32(b), bit for bit against the
+    oracle's FP32 restatement.  Default blocks run the LDS sweep with the t slice in LDS;
+    two blocks of 16,392 rows run the global-memory sweep in FP32."""
     nx = 32
     row, col, val = xamg.poisson7(nx)
     n = nx ** 3
     b = xamg.rhs(n, nv, 7)
+    solver = {"method": "HSGS", "max_iters": 2, "relax_factor": 0.9,
+              "convergence_details": "nocheck"}
+    if blocks is not None:
+        solver["gs_blocks"] = blocks
     params = {"meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
                               "max_iters": 1, "rel_tolerance": 1e-30, "abs_tolerance": 0.0},
-              "solver": {"method": "HSGS", "max_iters": 2, "relax_factor": 0.9,
-                         "convergence_details": "nocheck"}}
+              "solver": solver}
     gpu = xamg.solve_system(row, col, val, b, nv, params)
-    orc = O.solve(dict(params, solver=dict(params["solver"], gs_blocks=n // 128)),
+    orc = O.solve(dict(params, solver=dict(solver, gs_blocks=blocks or n // 128)),
                   O.csr(row, col, val), b, nv)
     xg, xo = np.asarray(gpu["x"]), np.asarray(orc["x"])
     bad = np.flatnonzero(xg.view(np.uint64) != xo.view(np.uint64))

@@ -435,10 +435,10 @@ void spmv(DevMatrix &A, Prec vp, int epi, int nd, int nv, SpmvArgs<T> a, const R
     if (kt.on()) {
         char cls[160];
         // one class per compiled kernel and matrix (k_spmv<V, T, nv, vpt, epilogue, dots>)
-        snprintf(cls, sizeof(cls), "spmv %s/%s %s nd%d nv%d stage%d vpt%d n%lld", prec_tag(vp),
+        snprintf(cls, sizeof(cls), "spmv %s/%s %s nd%d nv%d stage%d vpt%d n%lld%s%s", prec_tag(vp),
                  sizeof(T) == 8 ? "f64" : "f32", epi_tag(epi), nd, nv, a.cfg_mode,
                  eff_vpt(nv, a.cfg_vpt > 0 ? a.cfg_vpt : spmv_vpt_choice(nv, (int)sizeof(T))),
-                 (long long)A.n);
+                 (long long)A.n, A.label.empty() ? "" : " ", A.label.c_str());
         tok = kt.begin(cls, spmv_launch_bytes<T>(A, vp, epi, nd, nv, a));
     }
     try {

@@ -10,6 +10,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 namespace xamg {
@@ -91,6 +92,9 @@ struct DevMatrix {
     // AMG hierarchy attached by the caller; MultiGrid solvers on this matrix use it
     // instead of running the setup.
     std::shared_ptr<MgHierarchy> hier;
+    // appended to the per-launch timer's class names of this matrix's SpMVs (a matrix a
+    // solver derives from another of the same size, such as HSGS's off-block part)
+    std::string label;
 
     void upload();
     const void *values(Prec p);

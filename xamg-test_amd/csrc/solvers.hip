@@ -589,6 +589,11 @@ class HSGS : public Solver<T> {
     int nblocks = 1;
     HsgsPlan plan;
     DBuf s_ec, s_ev, s_rc; // in-block row slots of the LDS sweep
+    // The off-block entries as a CSR of their own (same rows, ascending columns): t = b -
+    // A_offd x is then an SpMV with the residual epilogue on the autotuned launch shape.
+    // Its row sums add the same terms in the same order as a pass over A that skips the
+    // in-block entries, so t has the same bits; null when no entry lies off the blocks.
+    std::unique_ptr<DevMatrix> aoff;
 
   public:
     void setup() override {
@@ -620,6 +625,31 @@ class HSGS : public Solver<T> {
             plan.ev = s_ev.p;
             plan.rc = s_rc.as<int>();
         }
+        aoff.reset();
+        if (A->n > 0 && plan.bsize > 0) {
+            auto m = std::make_unique<DevMatrix>();
+            m->n = A->n;
+            m->ncols = A->ncols;
+            m->label = "offd";
+            m->hrow.assign(A->n + 1, 0);
+            for (int64_t i = 0; i < A->n; i++) {
+                const int64_t b0 = (i / plan.bsize) * plan.bsize;
+                const int64_t b1 = std::min<int64_t>(b0 + plan.bsize, A->n);
+                for (uint32_t j = A->hrow[i]; j < A->hrow[i + 1]; j++) {
+                    const int64_t c = A->hcol[j];
+                    if (c < b0 || c >= b1) {
+                        m->hcol.push_back(A->hcol[j]);
+                        m->hval.push_back(A->hval[j]);
+                    }
+                }
+                m->hrow[i + 1] = (uint32_t)m->hcol.size();
+            }
+            m->nnz = (int64_t)m->hcol.size();
+            if (m->nnz > 0) {
+                m->upload();
+                aoff = std::move(m);
+            }
+        }
     }
     void sweep(T *x, const T *b, const int *H, const T *mask) {
         auto *A = this->A;
@@ -637,14 +667,22 @@ class HSGS : public Solver<T> {
         const bool lds = hsgs_uses_lds(plan, nv, (int)sizeof(T));
         char cls[96];
         int tok = -1;
-        if (kt.on()) {
-            // row pointers, indices and values, x (gathered once), b read, t written
-            snprintf(cls, sizeof(cls), "hsgs_offd %s nv%d n%lld", tp, nv, (long long)A->n);
-            tok = kt.begin(cls, mat + 3 * E);
+        if (aoff) {
+            SpmvArgs<T> a{};
+            a.x = x;
+            a.y = t;
+            a.b = b;
+            spmv<T>(*aoff, this->vprec, EPI_RESID, 0, nv, a, red_args(H), FinArgs<T>{});
+        } else {
+            if (kt.on()) {
+                // row pointers, indices and values, x (gathered once), b read, t written
+                snprintf(cls, sizeof(cls), "hsgs_offd %s nv%d n%lld", tp, nv, (long long)A->n);
+                tok = kt.begin(cls, mat + 3 * E);
+            }
+            launch_hsgs_offd<T, T>(row, col, val, A->n, nv, b, x, t, plan, H, s);
+            kt.end(tok);
+            tok = -1;
         }
-        launch_hsgs_offd<T, T>(row, col, val, A->n, nv, b, x, t, plan, H, s);
-        kt.end(tok);
-        tok = -1;
         if (kt.on()) {
             // x slice read and written, t and the inverse diagonal read, and the row slots
             // (LDS sweep) or the rows' entries (global sweep)
