@@ -784,13 +784,14 @@ __global__ void __launch_bounds__(64)
     }
 }
 
-// Bytes of LDS k_hsgs_sweep takes for one block: the x slice, the inverse diagonal, and
-// the in-block entries as rs slots per row (values, 16-bit local columns) with their
-// counts; in FP32 also the t slice.
+// Bytes of LDS k_hsgs_sweep takes for one block: the x slice and a row of zeros (the
+// column of the unused slots), the inverse diagonal, and the in-block entries as rs slots
+// per row (values, 16-bit local columns); in FP32 also the t slice.  Each part starts on
+// a 16-byte boundary.
 template <typename T>
 __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) {
-    size_t b = sizeof(T) * (size_t)(bsize * nv + bsize + bsize * rs);
-    b += 2 * (size_t)(bsize * rs) + 4 * (size_t)bsize;
+    size_t b = (sizeof(T) * (size_t)((bsize + 1) * nv) + 15) & ~(size_t)15;
+    b += sizeof(T) * (size_t)(bsize + bsize * rs) + 2 * (size_t)(bsize * rs);
     if (sizeof(T) < 8)
         b += sizeof(T) * (size_t)(bsize * nv); // FP32: the t slice too
     return (b + 15) & ~(size_t)15;
@@ -799,36 +800,40 @@ __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) 
 // In-block sweep from LDS: one wavefront per block.  The wave copies the block's x slice,
 // inverse diagonal and in-block row slots (HsgsPlan) to LDS with consecutive lanes on
 // consecutive elements.  The NV lanes then run the forward and backward relaxation: the
-// next row's slots and diagonal are read from LDS (at addresses that follow from the row
-// index alone) while the current row is relaxed, so each row waits for one round of LDS
-// reads, its x entries.  In FP64 each lane's t values come from global memory kHsgsTRing
-// rows ahead through a ring of registers (keeping t out of LDS lets more blocks share a
-// CU: 628 instead of 721 us per sweep at 128^3 nv=8); in FP32 the t slice is copied to LDS
-// too and read with the row's slots (the global ring made the FP32 sweep slower, 397
-// against 376 us).  The operations are those of the global-memory loop in the same order,
-// so the results are bit-identical.
+// next row's slots and diagonal are read from LDS while the current row is relaxed (two
+// register sets alternate), so each row waits for one round of LDS reads, its x entries.
+// Every slot is added (unused ones add +0, see HsgsPlan), so a row step is straight-line
+// code.  The steps run in rounds of kHsgsTRing: a round whose rows and look-ahead rows all
+// lie in the forward (or all in the backward) half addresses them as constant offsets
+// from one base; the rounds around the turn and the last one compute each row.  In FP64
+// each lane's t values come from global memory kHsgsTRing rows ahead through a ring of
+// registers (keeping t out of LDS lets more blocks share a CU); in FP32 the t slice is
+// copied to LDS too and read with the row's slots.  The operations are those of the
+// global-memory loop in the same order, so the results are bit-identical.
 template <typename V, typename T, int NV, int RM>
 __global__ void __launch_bounds__(64)
     k_hsgs_sweep(const uint16_t *__restrict__ gec, const T *__restrict__ gev,
-                 const int *__restrict__ grc, const T *__restrict__ inv_diag, int64_t n,
-                 const T *__restrict__ t, T *__restrict__ x, const T *__restrict__ relax,
-                 const T *__restrict__ iconv, int64_t bsize, const int *halt) {
+                 const T *__restrict__ inv_diag, int64_t n, const T *__restrict__ t,
+                 T *__restrict__ x, const T *__restrict__ relax, const T *__restrict__ iconv,
+                 int64_t bsize, const int *halt) {
     constexpr int P = kHsgsTRing;
+    constexpr bool TG = sizeof(T) == 8; // t from global memory
+    static_assert(RM == 4 || RM == 8, "row slots come in fours or eights");
+    using Cols = typename std::conditional<RM == 4, u32x2_t, u32x4_t>::type; // RM u16 columns
     extern __shared__ __align__(16) unsigned char hsgs_smem[];
     if (halt != nullptr && *halt)
         return;
     const int64_t off = (int64_t)blockIdx.x * bsize;
     if (off >= n)
         return;
+    const int bs = (int)bsize;
     const int len = (int)(off + bsize < n ? bsize : n - off);
     const int lane = threadIdx.x;
-    T *xs = (T *)hsgs_smem;
-    T *dg = xs + bsize * NV;
-    T *ev = dg + bsize;                            // [bsize][RM] values
-    uint16_t *ec = (uint16_t *)(ev + bsize * RM);  // [bsize][RM] local columns
-    int *rc = (int *)(ec + bsize * RM);            // [bsize] in-block entries per row
-    constexpr bool TG = sizeof(T) == 8;            // t from global memory
-    T *ts = (T *)(rc + bsize);                     // [bsize][NV] t slice (FP32)
+    T *xs = (T *)hsgs_smem; // [bs + 1][NV]: the block's rows, then the zero row
+    T *dg = (T *)(hsgs_smem + ((sizeof(T) * (size_t)(bs + 1) * NV + 15) & ~(size_t)15));
+    T *ev = dg + bs;                                   // [bs][RM] values
+    uint16_t *ec = (uint16_t *)(ev + (size_t)bs * RM); // [bs][RM] local columns
+    T *ts = (T *)(ec + (size_t)bs * RM);               // [bs][NV] t slice (FP32)
 
     const int64_t off_e = off * NV, off_s = off * RM;
     for (int k = lane; k < len * NV; k += 64) {
@@ -836,95 +841,122 @@ __global__ void __launch_bounds__(64)
         if constexpr (!TG)
             ts[k] = t[off_e + k];
     }
+    if (lane < NV)
+        xs[bs * NV + lane] = (T)0;
     for (int k = lane; k < len * RM; k += 64) {
         ev[k] = gev[off_s + k];
         ec[k] = gec[off_s + k];
     }
-    for (int r = lane; r < len; r += 64) {
+    for (int r = lane; r < len; r += 64)
         dg[r] = inv_diag[off + r];
-        rc[r] = grc[off + r];
-    }
     __syncthreads();
     if (lane < NV) {
         const T w = iconv != nullptr ? relax[lane] * iconv[lane] : relax[lane];
         struct Row {
-            int cnt;
-            uint16_t c[RM];
+            Cols c;
             T v[RM], d, t;
         };
         auto fetch = [&](Row &r, int i) {
-            r.cnt = rc[i];
+            r.c = *(const Cols *)(ec + i * RM);
 #pragma unroll
-            for (int u = 0; u < RM; u++) {
-                r.c[u] = ec[i * RM + u];
+            for (int u = 0; u < RM; u++)
                 r.v[u] = ev[i * RM + u];
-            }
             r.d = dg[i];
             if constexpr (!TG)
                 r.t = ts[i * NV + lane];
         };
+        auto col = [](const Cols &c, int u) {
+            const uint32_t wd = c[u >> 1];
+            return (int)((u & 1) ? wd >> 16 : wd & 0xffffu);
+        };
         // relax row i (its t value tv) from its slots in cur: its x entries are read
-        // first, then the next row's slots into nxt (inext < 0: none), then the row sum
-        auto step = [&](const Row &cur, int i, T tv, Row &nxt, int inext) {
+        // first, then (more) the next row's slots into nxt, then the row sum
+        auto step = [&](const Row &cur, int i, T tv, Row &nxt, int inext, bool more) {
             T xv[RM];
 #pragma unroll
             for (int u = 0; u < RM; u++)
-                xv[u] = xs[(int)cur.c[u] * NV + lane];
+                xv[u] = xs[col(cur.c, u) * NV + lane];
             const T xi = xs[i * NV + lane];
-            if (inext >= 0)
+            if (more)
                 fetch(nxt, inext);
             T sum = tv;
 #pragma unroll
-            for (int u = 0; u < RM; u++) {
-                if (u >= cur.cnt)
-                    break;
+            for (int u = 0; u < RM; u++)
                 sum -= xv[u] * cur.v[u];
-            }
             const T coeff = w * cur.d;
             xs[i * NV + lane] = xi + coeff * sum;
         };
-        // forward then backward: step s relaxes row s, then row 2 len - 1 - s; the two
-        // slot sets alternate (P is even), so no registers are copied between steps
+        // forward then backward: step s relaxes row s, then row 2 len - 1 - s
         const int ns = 2 * len;
         auto rowof = [&](int s) { return s < len ? s : ns - 1 - s; };
         Row ra, rb;
         fetch(ra, 0);
+        T ring[P];
         if constexpr (TG) {
-            // the ring's loads take the row of step min(s + P, ns - 1), so they are
-            // issued without a branch; full rounds of P steps, then the remaining steps
-            auto tload = [&](int s) { return t[(off + rowof(s < ns ? s : ns - 1)) * NV + lane]; };
-            T ring[P];
+            // the ring's loads take the row of step min(s + P, ns - 1)
 #pragma unroll
             for (int k = 0; k < P; k++)
-                ring[k] = tload(k);
-            int s0 = 0;
-            for (; s0 + P <= ns; s0 += P) {
+                ring[k] = t[(off + rowof(k < ns ? k : ns - 1)) * NV + lane];
+        }
+        // one round of P steps from step s0; K = 1: every step and look-ahead row in the
+        // forward half, 2: all in the backward half, 0: rows computed per step
+        auto round = [&](int s0, auto kind) {
+            constexpr int K = decltype(kind)::value;
+            // first look-ahead element of the round (forward or backward half)
+            const int64_t tb = K == 1 ? (off + s0 + P) * NV + lane : (off + ns - 1 - s0 - P) * NV + lane;
+            const int ib = K == 1 ? s0 : ns - 1 - s0;
 #pragma unroll
-                for (int k = 0; k < P; k++) {
-                    const int s = s0 + k;
-                    const T tv = ring[k];
-                    ring[k] = tload(s + P);
-                    const int inext = s + 1 < ns ? rowof(s + 1) : -1;
-                    if (k & 1)
-                        step(rb, rowof(s), tv, ra, inext);
+            for (int k = 0; k < P; k++) {
+                const int s = s0 + k;
+                const int i = K == 1 ? ib + k : K == 2 ? ib - k : rowof(s);
+                const int inext = K == 1 ? i + 1 : K == 2 ? i - 1 : (s + 1 < ns ? rowof(s + 1) : -1);
+                T tv;
+                if constexpr (TG) {
+                    tv = ring[k];
+                    if constexpr (K == 1)
+                        ring[k] = t[tb + k * NV];
+                    else if constexpr (K == 2)
+                        ring[k] = t[tb - k * NV];
                     else
-                        step(ra, rowof(s), tv, rb, inext);
+                        ring[k] = t[(off + rowof(s + P < ns ? s + P : ns - 1)) * NV + lane];
+                } else {
+                    tv = (k & 1) ? rb.t : ra.t;
+                }
+                if (K != 0 || inext >= 0) {
+                    if (k & 1)
+                        step(rb, i, tv, ra, inext, true);
+                    else
+                        step(ra, i, tv, rb, inext, true);
+                } else {
+                    if (k & 1)
+                        step(rb, i, tv, ra, 0, false);
+                    else
+                        step(ra, i, tv, rb, 0, false);
                 }
             }
-            // fewer than P steps remain (an even number: ns and P are even)
+        };
+        int s0 = 0;
+        for (; s0 + P <= ns; s0 += P) {
+            if (s0 + 2 * P <= len)
+                round(s0, std::integral_constant<int, 1>{});
+            else if (s0 >= len && s0 + 2 * P <= ns)
+                round(s0, std::integral_constant<int, 2>{});
+            else
+                round(s0, std::integral_constant<int, 0>{});
+        }
+        // fewer than P steps remain (an even number: ns and P are even)
 #pragma unroll
-            for (int k = 0; k < P; k += 2) {
-                const int s = s0 + k;
-                if (s >= ns)
-                    break;
-                step(ra, rowof(s), ring[k], rb, rowof(s + 1));
-                step(rb, rowof(s + 1), ring[k + 1], ra, s + 2 < ns ? rowof(s + 2) : -1);
-            }
-        } else {
-            for (int s = 0; s < ns; s += 2) {
-                step(ra, rowof(s), ra.t, rb, rowof(s + 1));
-                step(rb, rowof(s + 1), rb.t, ra, s + 2 < ns ? rowof(s + 2) : -1);
-            }
+        for (int k = 0; k < P; k += 2) {
+            const int s = s0 + k;
+            if (s >= ns)
+                break;
+            const T t0 = TG ? ring[k] : ra.t;
+            step(ra, rowof(s), t0, rb, rowof(s + 1), true);
+            const T t1 = TG ? ring[k + 1] : rb.t;
+            if (s + 2 < ns)
+                step(rb, rowof(s + 1), t1, ra, rowof(s + 2), true);
+            else
+                step(rb, rowof(s + 1), t1, ra, 0, false);
         }
     }
     __syncthreads();
@@ -1065,12 +1097,13 @@ HsgsPlan hsgs_plan(const uint32_t *hrow, const uint32_t *hcol, int64_t n, int nb
 
 template <typename T>
 void hsgs_slots(const HsgsPlan &p, const uint32_t *hrow, const uint32_t *hcol,
-                const double *hval, int64_t n, std::vector<uint16_t> &ec, std::vector<T> &ev,
-                std::vector<int> &rc) {
+                const double *hval, int64_t n, std::vector<uint16_t> &ec, std::vector<T> &ev) {
     const int RM = p.rs;
-    ec.assign((size_t)n * RM, 0);
+    if (p.bsize > 65535)
+        throw std::runtime_error("xamg: HSGS row slots need blocks of at most 65535 rows");
+    // unused slots: value 0 at column bsize, the zero row of the sweep's LDS copy of x
+    ec.assign((size_t)n * RM, (uint16_t)p.bsize);
     ev.assign((size_t)n * RM, (T)0);
-    rc.assign((size_t)n, 0);
     for (int64_t off = 0; off < n; off += p.bsize) {
         const int64_t end = std::min(n, off + p.bsize);
         for (int64_t i = off; i < end; i++) {
@@ -1085,16 +1118,15 @@ void hsgs_slots(const HsgsPlan &p, const uint32_t *hrow, const uint32_t *hcol,
                 ev[(size_t)i * RM + q] = (T)hval[j];
                 q++;
             }
-            rc[i] = q;
         }
     }
 }
 template void hsgs_slots<double>(const HsgsPlan &, const uint32_t *, const uint32_t *,
                                  const double *, int64_t, std::vector<uint16_t> &,
-                                 std::vector<double> &, std::vector<int> &);
+                                 std::vector<double> &);
 template void hsgs_slots<float>(const HsgsPlan &, const uint32_t *, const uint32_t *,
                                 const double *, int64_t, std::vector<uint16_t> &,
-                                std::vector<float> &, std::vector<int> &);
+                                std::vector<float> &);
 
 template <typename V, typename T, int NV>
 static void hsgs_offd_nv(const uint32_t *row, const uint32_t *col, const V *val, int64_t n,
@@ -1113,10 +1145,10 @@ static void hsgs_sweep_nv(const uint32_t *row, const uint32_t *col, const V *val
     const size_t lds = hsgs_lds_bytes<T>(p.bsize, NV, p.rs);
     if (hsgs_uses_lds(p, NV, sizeof(T)) && p.rs == 4)
         XAMG_LAUNCH((k_hsgs_sweep<V, T, NV, 4>), dim3(p.nblocks), dim3(64), lds, s, p.ec,
-                    (const T *)p.ev, p.rc, inv_diag, n, t, x, relax, iconv, p.bsize, halt);
+                    (const T *)p.ev, inv_diag, n, t, x, relax, iconv, p.bsize, halt);
     else if (hsgs_uses_lds(p, NV, sizeof(T)))
         XAMG_LAUNCH((k_hsgs_sweep<V, T, NV, kHsgsRow>), dim3(p.nblocks), dim3(64), lds, s, p.ec,
-                    (const T *)p.ev, p.rc, inv_diag, n, t, x, relax, iconv, p.bsize, halt);
+                    (const T *)p.ev, inv_diag, n, t, x, relax, iconv, p.bsize, halt);
     else
         XAMG_LAUNCH((k_hsgs_diag<V, T, NV>), dim3(p.nblocks), dim3(64), 0, s, row, col, val,
                     inv_diag, n, t, x, relax, iconv, p.bsize, halt);

@@ -290,9 +290,12 @@ void launch_dense_mv(const TA *at, int64_t n, const T *x, T *y, int nv, const in
 // number of in-block entries of a row.  Built on the host from the CSR.
 // When every row has at most kHsgsRow in-block entries, those entries are also kept
 // as rs slots per row in device memory (rs = 4 when every row has at most 4, else
-// kHsgsRow; ec: column - block start, ev: value in the sweep's precision, unused slots
-// 0; rc: entries per row), which the LDS sweep copies per block; otherwise (rs = 0, or
-// when a block's copies exceed kHsgsLdsMax bytes) the sweep runs from global memory.
+// kHsgsRow; ec: column - block start, ev: value in the sweep's precision, in the row's
+// column order), which the LDS sweep copies per block; a row's unused slots hold value 0
+// and column bsize, a row of zeros in the sweep's LDS copy of x, so x_c * a = +0 and
+// sum - (+0) = sum for every sum: every slot is added, with no per-row count or branch.
+// Otherwise (rs = 0, or when a block's copies exceed kHsgsLdsMax bytes) the sweep runs
+// from global memory.
 struct HsgsPlan {
     int64_t bsize = 0;
     int nblocks = 0;
@@ -300,7 +303,6 @@ struct HsgsPlan {
     int rs = 0;
     const uint16_t *ec = nullptr;
     const void *ev = nullptr;
-    const int *rc = nullptr;
 };
 constexpr int kHsgsRow = 8;
 constexpr size_t kHsgsLdsMax = 64 * 1024;
@@ -310,8 +312,7 @@ HsgsPlan hsgs_plan(const uint32_t *hrow, const uint32_t *hcol, int64_t n, int nb
 // Host copies of the row slots of plan p (values from hval, rounded to T).
 template <typename T>
 void hsgs_slots(const HsgsPlan &p, const uint32_t *hrow, const uint32_t *hcol,
-                const double *hval, int64_t n, std::vector<uint16_t> &ec, std::vector<T> &ev,
-                std::vector<int> &rc);
+                const double *hval, int64_t n, std::vector<uint16_t> &ec, std::vector<T> &ev);
 
 // Block-hybrid symmetric Gauss-Seidel sweep (csr_gs.inl:32-134 + blas2.h:225-263) in two
 // launches: t = b - A_offd x_old (the off-block residual, t is scratch of n * nv

@@ -588,7 +588,7 @@ class HSGS : public Solver<T> {
     DBuf relax;
     int nblocks = 1;
     HsgsPlan plan;
-    DBuf s_ec, s_ev, s_rc; // in-block row slots of the LDS sweep
+    DBuf s_ec, s_ev; // in-block row slots of the LDS sweep
     // The off-block entries as a CSR of their own (same rows, ascending columns): t = b -
     // A_offd x is then an SpMV with the residual epilogue on the autotuned launch shape.
     // Its row sums add the same terms in the same order as a pass over A that skips the
@@ -613,17 +613,13 @@ class HSGS : public Solver<T> {
         if (plain && plan.rs > 0 && plan.bsize <= 65535 && A->n > 0) {
             std::vector<uint16_t> ec;
             std::vector<T> ev;
-            std::vector<int> rc;
-            hsgs_slots<T>(plan, A->hrow.data(), A->hcol.data(), A->hval.data(), A->n, ec, ev, rc);
+            hsgs_slots<T>(plan, A->hrow.data(), A->hcol.data(), A->hval.data(), A->n, ec, ev);
             s_ec.alloc(sizeof(uint16_t) * ec.size());
             s_ev.alloc(sizeof(T) * ev.size());
-            s_rc.alloc(sizeof(int) * rc.size());
             XAMG_HIP_CHECK(hipMemcpy(s_ec.p, ec.data(), sizeof(uint16_t) * ec.size(), hipMemcpyHostToDevice));
             XAMG_HIP_CHECK(hipMemcpy(s_ev.p, ev.data(), sizeof(T) * ev.size(), hipMemcpyHostToDevice));
-            XAMG_HIP_CHECK(hipMemcpy(s_rc.p, rc.data(), sizeof(int) * rc.size(), hipMemcpyHostToDevice));
             plan.ec = s_ec.as<uint16_t>();
             plan.ev = s_ev.p;
-            plan.rc = s_rc.as<int>();
         }
         aoff.reset();
         if (A->n > 0 && plan.bsize > 0) {
