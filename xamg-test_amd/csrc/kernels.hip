@@ -786,14 +786,11 @@ __global__ void __launch_bounds__(64)
 
 // Bytes of LDS k_hsgs_sweep takes for one block: the x slice and a row of zeros (the
 // column of the unused slots), the inverse diagonal, and the in-block entries as rs slots
-// per row (values, 16-bit local columns); in FP32 also the t slice.  Each part starts on
-// a 16-byte boundary.
+// per row (values, 16-bit local columns).  Each part starts on a 16-byte boundary.
 template <typename T>
 __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) {
     size_t b = (sizeof(T) * (size_t)((bsize + 1) * nv) + 15) & ~(size_t)15;
     b += sizeof(T) * (size_t)(bsize + bsize * rs) + 2 * (size_t)(bsize * rs);
-    if (sizeof(T) < 8)
-        b += sizeof(T) * (size_t)(bsize * nv); // FP32: the t slice too
     return (b + 15) & ~(size_t)15;
 }
 
@@ -805,11 +802,11 @@ __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) 
 // Every slot is added (unused ones add +0, see HsgsPlan), so a row step is straight-line
 // code.  The steps run in rounds of kHsgsTRing: a round whose rows and look-ahead rows all
 // lie in the forward (or all in the backward) half addresses them as constant offsets
-// from one base; the rounds around the turn and the last one compute each row.  In FP64
-// each lane's t values come from global memory kHsgsTRing rows ahead through a ring of
-// registers (keeping t out of LDS lets more blocks share a CU); in FP32 the t slice is
-// copied to LDS too and read with the row's slots.  The operations are those of the
-// global-memory loop in the same order, so the results are bit-identical.
+// from one base; the rounds around the turn and the last one compute each row.  Each
+// lane's t values come from global memory kHsgsTRing rows ahead through a ring of
+// registers: t stays out of LDS, so more blocks share a CU (at 128^3 nv=8 the FP32 sweep
+// took 232 us this way against 263 us with the t slice in LDS).  The operations are those
+// of the global-memory loop in the same order, so the results are bit-identical.
 template <typename V, typename T, int NV, int RM>
 __global__ void __launch_bounds__(64)
     k_hsgs_sweep(const uint16_t *__restrict__ gec, const T *__restrict__ gev,
@@ -817,7 +814,6 @@ __global__ void __launch_bounds__(64)
                  T *__restrict__ x, const T *__restrict__ relax, const T *__restrict__ iconv,
                  int64_t bsize, const int *halt) {
     constexpr int P = kHsgsTRing;
-    constexpr bool TG = sizeof(T) == 8; // t from global memory
     static_assert(RM == 4 || RM == 8, "row slots come in fours or eights");
     using Cols = typename std::conditional<RM == 4, u32x2_t, u32x4_t>::type; // RM u16 columns
     extern __shared__ __align__(16) unsigned char hsgs_smem[];
@@ -833,14 +829,10 @@ __global__ void __launch_bounds__(64)
     T *dg = (T *)(hsgs_smem + ((sizeof(T) * (size_t)(bs + 1) * NV + 15) & ~(size_t)15));
     T *ev = dg + bs;                                   // [bs][RM] values
     uint16_t *ec = (uint16_t *)(ev + (size_t)bs * RM); // [bs][RM] local columns
-    T *ts = (T *)(ec + (size_t)bs * RM);               // [bs][NV] t slice (FP32)
 
     const int64_t off_e = off * NV, off_s = off * RM;
-    for (int k = lane; k < len * NV; k += 64) {
+    for (int k = lane; k < len * NV; k += 64)
         xs[k] = x[off_e + k];
-        if constexpr (!TG)
-            ts[k] = t[off_e + k];
-    }
     if (lane < NV)
         xs[bs * NV + lane] = (T)0;
     for (int k = lane; k < len * RM; k += 64) {
@@ -854,7 +846,7 @@ __global__ void __launch_bounds__(64)
         const T w = iconv != nullptr ? relax[lane] * iconv[lane] : relax[lane];
         struct Row {
             Cols c;
-            T v[RM], d, t;
+            T v[RM], d;
         };
         auto fetch = [&](Row &r, int i) {
             r.c = *(const Cols *)(ec + i * RM);
@@ -862,8 +854,6 @@ __global__ void __launch_bounds__(64)
             for (int u = 0; u < RM; u++)
                 r.v[u] = ev[i * RM + u];
             r.d = dg[i];
-            if constexpr (!TG)
-                r.t = ts[i * NV + lane];
         };
         auto col = [](const Cols &c, int u) {
             const uint32_t wd = c[u >> 1];
@@ -891,13 +881,11 @@ __global__ void __launch_bounds__(64)
         auto rowof = [&](int s) { return s < len ? s : ns - 1 - s; };
         Row ra, rb;
         fetch(ra, 0);
+        // the ring's loads take the row of step min(s + P, ns - 1)
         T ring[P];
-        if constexpr (TG) {
-            // the ring's loads take the row of step min(s + P, ns - 1)
 #pragma unroll
-            for (int k = 0; k < P; k++)
-                ring[k] = t[(off + rowof(k < ns ? k : ns - 1)) * NV + lane];
-        }
+        for (int k = 0; k < P; k++)
+            ring[k] = t[(off + rowof(k < ns ? k : ns - 1)) * NV + lane];
         // one round of P steps from step s0; K = 1: every step and look-ahead row in the
         // forward half, 2: all in the backward half, 0: rows computed per step
         auto round = [&](int s0, auto kind) {
@@ -910,18 +898,13 @@ __global__ void __launch_bounds__(64)
                 const int s = s0 + k;
                 const int i = K == 1 ? ib + k : K == 2 ? ib - k : rowof(s);
                 const int inext = K == 1 ? i + 1 : K == 2 ? i - 1 : (s + 1 < ns ? rowof(s + 1) : -1);
-                T tv;
-                if constexpr (TG) {
-                    tv = ring[k];
-                    if constexpr (K == 1)
-                        ring[k] = t[tb + k * NV];
-                    else if constexpr (K == 2)
-                        ring[k] = t[tb - k * NV];
-                    else
-                        ring[k] = t[(off + rowof(s + P < ns ? s + P : ns - 1)) * NV + lane];
-                } else {
-                    tv = (k & 1) ? rb.t : ra.t;
-                }
+                const T tv = ring[k];
+                if constexpr (K == 1)
+                    ring[k] = t[tb + k * NV];
+                else if constexpr (K == 2)
+                    ring[k] = t[tb - k * NV];
+                else
+                    ring[k] = t[(off + rowof(s + P < ns ? s + P : ns - 1)) * NV + lane];
                 if (K != 0 || inext >= 0) {
                     if (k & 1)
                         step(rb, i, tv, ra, inext, true);
@@ -933,6 +916,14 @@ __global__ void __launch_bounds__(64)
                     else
                         step(ra, i, tv, rb, 0, false);
                 }
+                // FP32: keep each step's look-ahead load in its step (global memory
+                // instructions do not cross this point; ALU and LDS instructions may):
+                // left to itself the scheduler may sink a round's loads to its end, and
+                // the next round then waits for loads issued just before it (at 128^3
+                // nv=8 221 against 232 us per sweep; the FP64 sweep measured 448 against
+                // 379 us with the barrier, so it goes without)
+                if constexpr (sizeof(T) == 4)
+                    __builtin_amdgcn_sched_barrier(0x078F);
             }
         };
         int s0 = 0;
@@ -950,13 +941,11 @@ __global__ void __launch_bounds__(64)
             const int s = s0 + k;
             if (s >= ns)
                 break;
-            const T t0 = TG ? ring[k] : ra.t;
-            step(ra, rowof(s), t0, rb, rowof(s + 1), true);
-            const T t1 = TG ? ring[k + 1] : rb.t;
+            step(ra, rowof(s), ring[k], rb, rowof(s + 1), true);
             if (s + 2 < ns)
-                step(rb, rowof(s + 1), t1, ra, rowof(s + 2), true);
+                step(rb, rowof(s + 1), ring[k + 1], ra, rowof(s + 2), true);
             else
-                step(rb, rowof(s + 1), t1, ra, 0, false);
+                step(rb, rowof(s + 1), ring[k + 1], ra, 0, false);
         }
     }
     __syncthreads();
