@@ -788,9 +788,13 @@ __global__ void __launch_bounds__(64)
 // column of the unused slots), the inverse diagonal, and the in-block entries as rs slots
 // per row (values, 16-bit local columns).  Each part starts on a 16-byte boundary.
 template <typename T>
+constexpr bool hsgs_d_in_lds() {
+    return sizeof(T) < 8;
+}
+template <typename T>
 __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) {
     size_t b = (sizeof(T) * (size_t)((bsize + 1) * nv) + 15) & ~(size_t)15;
-    b += sizeof(T) * (size_t)(bsize + bsize * rs) + 2 * (size_t)(bsize * rs);
+    b += sizeof(T) * (size_t)((hsgs_d_in_lds<T>() ? bsize : 0) + bsize * rs) + 2 * (size_t)(bsize * rs);
     return (b + 15) & ~(size_t)15;
 }
 
@@ -826,8 +830,9 @@ __global__ void __launch_bounds__(64)
     const int len = (int)(off + bsize < n ? bsize : n - off);
     const int lane = threadIdx.x;
     T *xs = (T *)hsgs_smem; // [bs + 1][NV]: the block's rows, then the zero row
+    constexpr bool DL = hsgs_d_in_lds<T>(); // inverse diagonal in LDS, else in a ring
     T *dg = (T *)(hsgs_smem + ((sizeof(T) * (size_t)(bs + 1) * NV + 15) & ~(size_t)15));
-    T *ev = dg + bs;                                   // [bs][RM] values
+    T *ev = dg + (DL ? bs : 0);                        // [bs][RM] values
     uint16_t *ec = (uint16_t *)(ev + (size_t)bs * RM); // [bs][RM] local columns
 
     const int64_t off_e = off * NV, off_s = off * RM;
@@ -839,8 +844,9 @@ __global__ void __launch_bounds__(64)
         ev[k] = gev[off_s + k];
         ec[k] = gec[off_s + k];
     }
-    for (int r = lane; r < len; r += 64)
-        dg[r] = inv_diag[off + r];
+    if constexpr (DL)
+        for (int r = lane; r < len; r += 64)
+            dg[r] = inv_diag[off + r];
     __syncthreads();
     if (lane < NV) {
         const T w = iconv != nullptr ? relax[lane] * iconv[lane] : relax[lane];
@@ -853,7 +859,8 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int u = 0; u < RM; u++)
                 r.v[u] = ev[i * RM + u];
-            r.d = dg[i];
+            if constexpr (DL)
+                r.d = dg[i];
         };
         auto col = [](const Cols &c, int u) {
             const uint32_t wd = c[u >> 1];
@@ -861,7 +868,7 @@ __global__ void __launch_bounds__(64)
         };
         // relax row i (its t value tv) from its slots in cur: its x entries are read
         // first, then (more) the next row's slots into nxt, then the row sum
-        auto step = [&](const Row &cur, int i, T tv, Row &nxt, int inext, bool more) {
+        auto step = [&](const Row &cur, int i, T tv, T dv, Row &nxt, int inext, bool more) {
             T xv[RM];
 #pragma unroll
             for (int u = 0; u < RM; u++)
@@ -873,7 +880,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int u = 0; u < RM; u++)
                 sum -= xv[u] * cur.v[u];
-            const T coeff = w * cur.d;
+            const T coeff = w * (DL ? cur.d : dv);
             xs[i * NV + lane] = xi + coeff * sum;
         };
         // forward then backward: step s relaxes row s, then row 2 len - 1 - s
@@ -882,39 +889,52 @@ __global__ void __launch_bounds__(64)
         Row ra, rb;
         fetch(ra, 0);
         // the ring's loads take the row of step min(s + P, ns - 1)
-        T ring[P];
+        T ring[P], dring[P];
 #pragma unroll
-        for (int k = 0; k < P; k++)
-            ring[k] = t[(off + rowof(k < ns ? k : ns - 1)) * NV + lane];
+        for (int k = 0; k < P; k++) {
+            const int r = rowof(k < ns ? k : ns - 1);
+            ring[k] = t[(off + r) * NV + lane];
+            if constexpr (!DL)
+                dring[k] = inv_diag[off + r];
+        }
         // one round of P steps from step s0; K = 1: every step and look-ahead row in the
         // forward half, 2: all in the backward half, 0: rows computed per step
         auto round = [&](int s0, auto kind) {
             constexpr int K = decltype(kind)::value;
             // first look-ahead element of the round (forward or backward half)
-            const int64_t tb = K == 1 ? (off + s0 + P) * NV + lane : (off + ns - 1 - s0 - P) * NV + lane;
+            const int64_t rb0 = K == 1 ? off + s0 + P : off + ns - 1 - s0 - P;
+            const int64_t tb = rb0 * NV + lane;
             const int ib = K == 1 ? s0 : ns - 1 - s0;
 #pragma unroll
             for (int k = 0; k < P; k++) {
                 const int s = s0 + k;
                 const int i = K == 1 ? ib + k : K == 2 ? ib - k : rowof(s);
                 const int inext = K == 1 ? i + 1 : K == 2 ? i - 1 : (s + 1 < ns ? rowof(s + 1) : -1);
-                const T tv = ring[k];
-                if constexpr (K == 1)
+                const T tv = ring[k], dv = DL ? (T)0 : dring[k];
+                if constexpr (K == 1) {
                     ring[k] = t[tb + k * NV];
-                else if constexpr (K == 2)
+                    if constexpr (!DL)
+                        dring[k] = inv_diag[rb0 + k];
+                } else if constexpr (K == 2) {
                     ring[k] = t[tb - k * NV];
-                else
-                    ring[k] = t[(off + rowof(s + P < ns ? s + P : ns - 1)) * NV + lane];
+                    if constexpr (!DL)
+                        dring[k] = inv_diag[rb0 - k];
+                } else {
+                    const int r = rowof(s + P < ns ? s + P : ns - 1);
+                    ring[k] = t[(off + r) * NV + lane];
+                    if constexpr (!DL)
+                        dring[k] = inv_diag[off + r];
+                }
                 if (K != 0 || inext >= 0) {
                     if (k & 1)
-                        step(rb, i, tv, ra, inext, true);
+                        step(rb, i, tv, dv, ra, inext, true);
                     else
-                        step(ra, i, tv, rb, inext, true);
+                        step(ra, i, tv, dv, rb, inext, true);
                 } else {
                     if (k & 1)
-                        step(rb, i, tv, ra, 0, false);
+                        step(rb, i, tv, dv, ra, 0, false);
                     else
-                        step(ra, i, tv, rb, 0, false);
+                        step(ra, i, tv, dv, rb, 0, false);
                 }
                 // FP32: keep each step's look-ahead load in its step (global memory
                 // instructions do not cross this point; ALU and LDS instructions may):
@@ -941,11 +961,12 @@ __global__ void __launch_bounds__(64)
             const int s = s0 + k;
             if (s >= ns)
                 break;
-            step(ra, rowof(s), ring[k], rb, rowof(s + 1), true);
+            const T d0 = DL ? (T)0 : dring[k], d1 = DL ? (T)0 : dring[k + 1];
+            step(ra, rowof(s), ring[k], d0, rb, rowof(s + 1), true);
             if (s + 2 < ns)
-                step(rb, rowof(s + 1), ring[k + 1], ra, rowof(s + 2), true);
+                step(rb, rowof(s + 1), ring[k + 1], d1, ra, rowof(s + 2), true);
             else
-                step(rb, rowof(s + 1), ring[k + 1], ra, 0, false);
+                step(rb, rowof(s + 1), ring[k + 1], d1, ra, 0, false);
         }
     }
     __syncthreads();
