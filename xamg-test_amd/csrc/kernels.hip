@@ -785,8 +785,9 @@ __global__ void __launch_bounds__(64)
 }
 
 // Bytes of LDS k_hsgs_sweep takes for one block: the x slice and a row of zeros (the
-// column of the unused slots), the inverse diagonal, and the in-block entries as rs slots
-// per row (values, 16-bit local columns).  Each part starts on a 16-byte boundary.
+// column of the unused slots), the inverse diagonal (FP32; FP64 reads it through a
+// register ring), and the in-block entries as rs slots per row (values, 16-bit local
+// columns).  Each part starts on a 16-byte boundary.
 template <typename T>
 constexpr bool hsgs_d_in_lds() {
     return sizeof(T) < 8;
@@ -808,9 +809,11 @@ __host__ __device__ inline size_t hsgs_lds_bytes(int64_t bsize, int nv, int rs) 
 // lie in the forward (or all in the backward) half addresses them as constant offsets
 // from one base; the rounds around the turn and the last one compute each row.  Each
 // lane's t values come from global memory kHsgsTRing rows ahead through a ring of
-// registers: t stays out of LDS, so more blocks share a CU (at 128^3 nv=8 the FP32 sweep
-// took 232 us this way against 263 us with the t slice in LDS).  The operations are those
-// of the global-memory loop in the same order, so the results are bit-identical.
+// registers, and in FP64 the rows' inverse diagonal too: they stay out of LDS, so more
+// blocks share a CU (at 128^3 nv=8 the FP32 sweep took 232 us this way against 263 us
+// with the t slice in LDS; the FP64 one 367 against 380 us with the diagonal in LDS).
+// The operations are those of the global-memory loop in the same order, so the results
+// are bit-identical.
 template <typename V, typename T, int NV, int RM>
 __global__ void __launch_bounds__(64)
     k_hsgs_sweep(const uint16_t *__restrict__ gec, const T *__restrict__ gev,

@@ -684,7 +684,7 @@ class HSGS : public Solver<T> {
             // (LDS sweep) or the rows' entries (global sweep)
             snprintf(cls, sizeof(cls), "hsgs_sweep %s %s nv%d n%lld", tp, lds ? "lds" : "global",
                      nv, (long long)A->n);
-            const double rows = lds ? n * plan.rs * (2 + st) + 4 * n : mat;
+            const double rows = lds ? n * plan.rs * (2 + st) : mat;
             tok = kt.begin(cls, 3 * E + n * st + rows);
         }
         launch_hsgs_sweep<T, T>(row, col, val, d, A->n, nv, x, t, relax.as<T>(), mask, plan, H, s);
