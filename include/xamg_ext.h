@@ -86,7 +86,10 @@ int xamgx_kernel_timing(int on);
 /* Number of launch classes recorded (waits for the device). */
 int xamgx_kernel_stats_count(int *count);
 /* Class i: name (NUL-terminated, at most len bytes), launches, total device ms, total
- * algorithmic bytes.  NULL skips an output. */
+ * algorithmic bytes.  NULL skips an output.  Launches that returned at their flags
+ * without doing their work (queued behind a converged solver, or gated off), taken to
+ * be those that finished faster than their algorithmic bytes at 30 TB/s, are left out
+ * of all three. */
 int xamgx_kernel_stats_get(int i, char *name, int len, int64_t *calls, double *ms, double *bytes);
 /* Device time (ms) of the last solve (first kernel to the convergence readback). */
 int xamgx_solver_last_ms(XAMG_solver_id sid, double *ms);

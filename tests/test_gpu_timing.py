@@ -80,3 +80,28 @@ def test_kernel_timing_chebyshev_classes(xamg):
     # scaled by d), b and the scaling d and write y
     assert first[0]["bytes"] == pytest.approx(k * (mat + 2 * E + n * 8), rel=1e-12)
     assert mid[0]["bytes"] == pytest.approx(k * (mat + 3 * E + n * 8), rel=1e-12)
+
+
+def test_kernel_timing_leaves_out_no_op_launches(xamg):
+    """Launches queued behind a converged solver return at the halt flag; the timer leaves
+    them out (they finish far faster than their bytes could move), so a class counts the
+    launches that did their work: one v = A p per iteration."""
+    m, nv = 96, 16
+    row, col, val = X.poisson7(m)
+    n = m ** 3
+    b = X.rhs(n, nv, 4)
+    params = {"solver": {"method": "BiCGStab", "rel_tolerance": 1e-4, "abs_tolerance": 0.0,
+                         "max_iters": 500}}
+    X.set_engine(use_graphs=0)  # replayed graphs are not timed
+    X.kernel_timing(True)
+    out = xamg.solve_system(row, col, val, b, nv, params)
+    X.kernel_timing(False)
+    it = out["iters"]
+    assert 0 < it < 500
+    st = {k["name"]: k for k in X.kernel_stats()}
+    # (the in-solve launch choice tries several launch shapes, one class each)
+    ax1 = [v for k, v in st.items() if k.startswith("spmv ") and " axy nd1 " in k]
+    assert ax1, list(st)
+    assert sum(v["calls"] for v in ax1) == it
+    # what is left is real work: below 8 TB/s per launch on the algorithmic bytes
+    assert all(v["gbs"] < 8000 for v in ax1)

@@ -312,9 +312,14 @@ const std::vector<KernelTimer::Stat> &KernelTimer::stats() {
             float ms = 0;
             XAMG_HIP_CHECK(hipEventElapsedTime(&ms, p.e0, p.e1));
             Stat &s = st_[p.stat];
-            s.calls++;
-            s.ms += ms;
-            s.bytes += p.bytes;
+            if (p.bytes > 0 && (double)ms * 1e-3 * kNoWorkRate < p.bytes) {
+                s.skipped++;
+                s.skipped_ms += ms;
+            } else {
+                s.calls++;
+                s.ms += ms;
+                s.bytes += p.bytes;
+            }
             pool_.push_back(p.e0);
             pool_.push_back(p.e1);
         }

@@ -201,11 +201,16 @@ float bench_spmv_epi(DevMatrix &A, Prec vp, int nv, int epi, int nd, int cold, i
 // recorded while the stream is being captured into a graph.
 class KernelTimer {
   public:
+    // calls, ms and bytes cover the launches that did their work; a launch that returned
+    // at its flags (queued behind a converged solver, or gated off) is counted in skipped
+    // and skipped_ms instead.  A launch is taken to have returned so when it finished
+    // faster than its algorithmic bytes could move at kNoWorkRate.
     struct Stat {
         std::string name;
-        int64_t calls = 0;
-        double ms = 0, bytes = 0;
+        int64_t calls = 0, skipped = 0;
+        double ms = 0, bytes = 0, skipped_ms = 0;
     };
+    static constexpr double kNoWorkRate = 30e12; // bytes/s, ~4x the HBM peak
     static KernelTimer &get();
     bool on() const { return on_; }
     // stops recording for its lifetime (launches of the autotuner and probes)
