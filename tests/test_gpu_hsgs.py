@@ -103,8 +103,8 @@ def test_hsgs_ragged_rows(xamg, blocks):
 def test_hsgs_fp32_inner_bit_exact(xamg, nv, blocks):
     """FP32 HSGS sweeps as the inner solver of one IterativeRefinement step from x0 = 0
     (ir.inl:107-194): x = (double) of two FP32 sweeps on fl32(b), bit for bit against the
-    oracle's FP32 restatement.  Default blocks run the LDS sweep with the t slice in LDS;
-    two blocks of 16,392 rows run the global-memory sweep in FP32."""
+    oracle's FP32 restatement.  Default blocks run the LDS sweep; two blocks of 16,392
+    rows run the global-memory sweep in FP32."""
     nx = 32
     row, col, val = xamg.poisson7(nx)
     n = nx ** 3
@@ -125,3 +125,49 @@ def test_hsgs_fp32_inner_bit_exact(xamg, nv, blocks):
     # the result is an FP32 solve widened to FP64
     assert np.array_equal(xg.astype(np.float32).astype(np.float64), xg)
     assert np.abs(xg).max() > 0
+
+
+def _band7(n, seed=5):
+    """1D 7-point band (offsets -3..3), diagonally dominant, random off-diagonals: rows
+    have up to 7 entries inside their block, so the LDS sweep takes 8 slots per row and
+    rows near a block edge pad several of them."""
+    rng = np.random.default_rng(seed)
+    row = np.zeros(n + 1, dtype=np.int64)
+    col, val = [], []
+    for i in range(n):
+        cs = [j for j in range(i - 3, i + 4) if 0 <= j < n]
+        vs = [(-rng.uniform(0.1, 1.0) if j != i else 0.0) for j in cs]
+        d = cs.index(i)
+        vs[d] = 1.0 + sum(abs(v) for v in vs)
+        col += cs
+        val += vs
+        row[i + 1] = len(col)
+    return row, np.asarray(col, dtype=np.int64), np.asarray(val)
+
+
+@pytest.mark.parametrize("nv", [2, 8])
+def test_hsgs_eight_slot_lds_sweep(xamg, nv):
+    """1D 7-point band, 4096 rows, default blocks of 136 rows: the LDS sweep with 8 slots per
+    row (up to 7 used)."""
+    n = 4096
+    row, col, val = _band7(n)
+    _check(xamg, row, col, val, nv, gs_blocks=n // 128, gpu_blocks=None, sweeps=3)
+
+
+def test_hsgs_eight_slot_fp32_inner(xamg):
+    """The same band with FP32 sweeps (8 slots) as the inner solver of one
+    IterativeRefinement step, bit for bit."""
+    n, nv = 4096, 8
+    row, col, val = _band7(n)
+    b = xamg.rhs(n, nv, 9)
+    solver = {"method": "HSGS", "max_iters": 2, "relax_factor": 0.9,
+              "convergence_details": "nocheck"}
+    params = {"meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
+                              "max_iters": 1, "rel_tolerance": 1e-30, "abs_tolerance": 0.0},
+              "solver": solver}
+    gpu = xamg.solve_system(row, col, val, b, nv, params)
+    orc = O.solve(dict(params, solver=dict(solver, gs_blocks=n // 128)),
+                  O.csr(row, col, val), b, nv)
+    xg, xo = np.asarray(gpu["x"]), np.asarray(orc["x"])
+    bad = np.flatnonzero(xg.view(np.uint64) != xo.view(np.uint64))
+    assert bad.size == 0, (bad[:8], xg[bad[:4]], xo[bad[:4]])
