@@ -99,7 +99,7 @@ def test_hsgs_ragged_rows(xamg, blocks):
            gpu_blocks=blocks)
 
 
-@pytest.mark.parametrize("nv,blocks", [(1, None), (8, None), (4, 2)])
+@pytest.mark.parametrize("nv,blocks", [(1, None), (2, None), (8, None), (16, None), (4, 2)])
 def test_hsgs_fp32_inner_bit_exact(xamg, nv, blocks):
     """FP32 HSGS sweeps as the inner solver of one IterativeRefinement step from x0 = 0
     (ir.inl:107-194): x = (double) of two FP32 sweeps on fl32(b), bit for bit against the
