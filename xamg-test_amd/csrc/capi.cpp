@@ -508,6 +508,10 @@ int XAMG_vector_upload_v(XAMG_vector_id in, XAMG_vector_id out) {
         size_t bytes = sizeof(double) * a->size * a->nv;
         double *dst = b->on_device() ? b->dev.as<double>() + a->ext_offset * a->nv
                                      : b->host.data() + a->ext_offset * a->nv;
+        // the copies run on the null stream, which does not wait for the library's
+        // non-blocking stream: finish the kernels that may still read or write either vector
+        if (a->on_device() || b->on_device())
+            Runtime::get().sync();
         if (a->on_device() && b->on_device())
             XAMG_HIP_CHECK(hipMemcpy(dst, a->dev.p, bytes, hipMemcpyDeviceToDevice));
         else if (!a->on_device() && b->on_device())
