@@ -69,6 +69,35 @@ def test_staged_bf16_27point(staged, nv):
     A.destroy()
 
 
+@pytest.mark.parametrize("nv", [1, 2, 4, 8, 16])
+def test_staged_27point_fp64_fp32_bitexact(staged, nv):
+    # long rows (27 nonzeros, fewer on the faces) with FP64 and FP32 values at every
+    # compiled width: tiles of 64-256 rows hold 1.7-6.9 thousand nonzeros, so some widths
+    # stage their tiles (modes 7 and 8 hold 4096 nonzeros, the others 2048) and some read
+    # them directly; y = A x and y += A x
+    X = staged
+    row, col, val = X.aniso27(12, 10, 9)
+    A = X.Matrix(row, col, val)
+    n = A.n
+    x = X.rhs(n, nv, 11)
+    ref64 = O.spmv(O.csr(row, col, val), x, nv)
+    x32 = x.astype(np.float32)
+    ref32 = O.spmv(O.csr(row, col, val.astype(np.float32)), x32, nv)
+    y0 = X.rhs(n, nv, 13)
+    refax = O.spmv(O.csr(row, col, val), x, nv, axpy_into=y0.copy())
+    for vpt in VPTS[nv]:
+        print(f"27-point nv={nv} vpt={vpt}", flush=True)
+        X.set_spmv_vpt(nv, 8, vpt)
+        X.set_spmv_vpt(nv, 4, vpt)
+        y = A.spmv(x, nv, prec=0)
+        assert np.array_equal(y.view(np.uint64), ref64.view(np.uint64)), vpt
+        y = A.spmv(x32, nv, prec=1)
+        assert np.array_equal(y.view(np.uint32), ref32.view(np.uint32)), vpt
+        y = A.spmv(x, nv, prec=0, y=y0, mode=1)
+        assert np.array_equal(y.view(np.uint64), refax.view(np.uint64)), vpt
+    A.destroy()
+
+
 def test_staged_ragged_long_and_empty_rows(staged):
     # rows of 0..9 nonzeros with a few rows of 1500-3000: some tiles exceed the LDS
     # capacity and are read directly, their neighbours are staged
