@@ -180,6 +180,44 @@ def test_long_rows_solve_identical_iterates_and_oracle(xamg, prec, nv):
     _compare(a, orc, nv, per)
 
 
+@pytest.mark.parametrize("nv", [4, 8])
+def test_long_rows_scaled_chebyshev_refinement_identical_iterates(xamg, nv):
+    """27-point rows through the FP32 Chebyshev steps on the scaled operator (the first
+    step gathers its right-hand side and the column scaling, the others gather x as it
+    is) inside continued FP64 refinement, plus the FP64 residual SpMVs: every forced
+    staging mode gives the same iterates and the same FP64 solution bit for bit, which is
+    verified in FP64 on the host."""
+    row, col, val = xamg.aniso27(18, 17, 16)
+    n = len(row) - 1
+    b = xamg.rhs(n, nv, 6)
+    params = {
+        "meta_solver": {"method": "IterativeRefinement", "subsolver_reduced_precision": True,
+                        "refinement": "continued", "max_iters": 50, "rel_tolerance": 1e-10,
+                        "abs_tolerance": 0.0, "convergence_details": "check-L2"},
+        "solver": {"method": "PBiCGStab", "rel_tolerance": 1e-4, "abs_tolerance": 0.0,
+                   "max_iters": 2000, "convergence_details": "check-L2"},
+        "preconditioner": {"method": "Chebyshev", "polynomial_order": 4,
+                           "spectrum_fraction": 0.3},
+    }
+    outs = []
+    try:
+        for mode in (-1, 0, 2, 3, 4, 5, 6, 7, 8):
+            print(f"27-point scaled Chebyshev refinement nv={nv} mode={mode}", flush=True)
+            xamg.set_spmv_stage(mode)
+            outs.append(xamg.solve_system(row, col, val, b, nv, params))
+    finally:
+        xamg.set_spmv_stage(-1)
+    a = outs[0]
+    assert all(a["converged"])
+    for s in outs[1:]:
+        assert (a["iters"], a["sub_iters"]) == (s["iters"], s["sub_iters"])
+        assert np.array_equal(a["x"].view(np.uint64), s["x"].view(np.uint64))
+    ax = O.spmv(O.csr(row, col, val), a["x"], nv).reshape(n, nv)
+    B = b.reshape(n, nv)
+    rel = np.linalg.norm(B - ax, axis=0) / np.linalg.norm(B, axis=0)
+    assert rel.max() <= 1e-10, rel
+
+
 def test_staged_wide_matrix_falls_back(staged):
     # entries 40000 rows off the diagonal: no 16-bit offsets, modes 4/5 run as 2/3
     X = staged
